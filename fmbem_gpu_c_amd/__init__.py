@@ -1,0 +1,345 @@
+"""fmbem_gpu_c_amd -- Python shim over libfmbem.so, the MI355X (gfx950) FMM-BEM
+solver for 3-D Helmholtz acoustics.
+
+The C ABI (include/fmbem.h) is the product; this module binds it with ctypes
+so tests, benchmarks and torch code can drive it.  Device vectors are torch
+complex128 CUDA tensors (interleaved re/im, the layout of fmb_c64).  There
+is no fallback: if libfmbem.so is missing or a call fails, FmbError is
+raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmbem.so")
+
+FMB_FP64, FMB_MIXED = 0, 1
+PRULE_TRUNCNUM, PRULE_EXCESS, PRULE_EXPLICIT = 0, 1, 2
+RULE_COLLAPSED, RULE_GAUSS7 = 0, 1
+COAX_RR, COAX_SR = 0, 1
+
+
+class FmbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("fmbem error %d: %s" % (code, msg))
+        self.code = code
+
+
+class PlanParams(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_double),
+        ("alpha_re", ctypes.c_double),
+        ("alpha_im", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("p_rule", ctypes.c_int),
+        ("eps", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("p_levels", ctypes.c_int * 16),
+        ("p_min", ctypes.c_int),
+        ("quad_rule", ctypes.c_int),
+        ("quad_order", ctypes.c_int),
+        ("near_rule", ctypes.c_int),
+        ("near_order", ctypes.c_int),
+        ("near_ratio", ctypes.c_double),
+        ("nedge", ctypes.c_int),
+        ("nrhs_max", ctypes.c_int),
+    ]
+
+
+class GmresInfo(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int),
+        ("matvecs_fp64", ctypes.c_int),
+        ("matvecs_work", ctypes.c_int),
+        ("restarts", ctypes.c_int),
+        ("relres", ctypes.c_double * 64),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every entry is declared in include/fmbem.h
+_SIGS = {
+    "fmb_last_error": (ctypes.c_char_p, []),
+    "fmb_version": (ctypes.c_int, []),
+    "fmb_obj_count": (ctypes.c_int, [ctypes.c_char_p, _ip, _ip]),
+    "fmb_mesh_load_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "fmb_mesh_from_arrays": (ctypes.c_int, [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "fmb_mesh_icosphere": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "fmb_mesh_ellipsoid": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+    "fmb_mesh_orient_outward": (ctypes.c_int, [_vp, _ip]),
+    "fmb_mesh_sizes": (ctypes.c_int, [_vp, _ip, _ip]),
+    "fmb_mesh_export": (ctypes.c_int, [_vp, _dp, _ip]),
+    "fmb_mesh_destroy": (None, [_vp]),
+    "fmb_tree_build": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    "fmb_tree_info": (ctypes.c_int, [_vp, _ip, _ip, _dp, _dp]),
+    "fmb_tree_level": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "fmb_tree_num_sr": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "fmb_tree_elem_box": (ctypes.c_int, [_vp, _ip, _ip]),
+    "fmb_tree_pairs": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip]),
+    "fmb_tree_destroy": (None, [_vp]),
+    "fmb_trunc_num": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "fmb_rot_size": (ctypes.c_int, [ctypes.c_int]),
+    "fmb_coax_size": (ctypes.c_int, [ctypes.c_int]),
+    "fmb_gen_rot_mats": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _vp]),
+    "fmb_gen_coax_mats": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int,
+                                         ctypes.c_int, _vp]),
+    "fmb_plan_params_default": (None, [ctypes.POINTER(PlanParams)]),
+    "fmb_plan_create": (ctypes.c_int, [_vp, ctypes.POINTER(PlanParams), _vp, _vp, ctypes.POINTER(_vp)]),
+    "fmb_plan_orders": (ctypes.c_int, [_vp, _ip]),
+    "fmb_plan_memory": (ctypes.c_int, [_vp, _dp]),
+    "fmb_matvec": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
+    "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.POINTER(GmresInfo), _vp]),
+    "fmb_plan_destroy": (None, [_vp]),
+    "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
+    "fmb_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+}
+
+
+def lib():
+    """Load libfmbem.so (built in-tree by fmbem_gpu_c_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FmbError(-3, "libfmbem.so not built: run python -m fmbem_gpu_c_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FmbError(rc, lib().fmb_last_error().decode())
+    return rc
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _dev(t):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128
+            and t.is_contiguous()):
+        raise FmbError(-1, "expected a contiguous complex128 CUDA tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Mesh:
+    """Triangle mesh (fmb_mesh)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_obj(cls, path):
+        h = _vp()
+        _check(lib().fmb_mesh_load_obj(path.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, V, F):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.int32)
+        h = _vp()
+        _check(lib().fmb_mesh_from_arrays(_ptr(V, ctypes.c_double), V.shape[0],
+                                          _ptr(F, ctypes.c_int), F.shape[0], ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def icosphere(cls, level, radius):
+        h = _vp()
+        _check(lib().fmb_mesh_icosphere(level, radius, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def ellipsoid(cls, nu, a, b, c, lmin=2, lmax=8, rms=0.05, seed=0):
+        h = _vp()
+        _check(lib().fmb_mesh_ellipsoid(nu, a, b, c, lmin, lmax, rms, seed, ctypes.byref(h)))
+        return cls(h)
+
+    def orient_outward(self):
+        n = ctypes.c_int()
+        _check(lib().fmb_mesh_orient_outward(self.h, ctypes.byref(n)))
+        return n.value
+
+    def arrays(self):
+        nv, nf = ctypes.c_int(), ctypes.c_int()
+        _check(lib().fmb_mesh_sizes(self.h, ctypes.byref(nv), ctypes.byref(nf)))
+        V = np.zeros((nv.value, 3))
+        F = np.zeros((nf.value, 3), dtype=np.int32)
+        _check(lib().fmb_mesh_export(self.h, _ptr(V, ctypes.c_double), _ptr(F, ctypes.c_int)))
+        return V, F
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fmb_mesh_destroy(self.h)
+            self.h = None
+
+
+class Tree:
+    """Device Morton octree (fmb_tree)."""
+
+    def __init__(self, mesh, s=5, depth=0):
+        self.mesh = mesh
+        self.h = _vp()
+        _check(lib().fmb_tree_build(mesh.h, s, depth, _stream(), ctypes.byref(self.h)))
+
+    def info(self):
+        lmin, lmax = ctypes.c_int(), ctypes.c_int()
+        D = ctypes.c_double()
+        pmin = np.zeros(3)
+        _check(lib().fmb_tree_info(self.h, ctypes.byref(lmin), ctypes.byref(lmax),
+                                   ctypes.byref(D), _ptr(pmin, ctypes.c_double)))
+        return lmin.value, lmax.value, D.value, pmin
+
+    def level(self, l):
+        n = ctypes.c_int()
+        _check(lib().fmb_tree_level(self.h, l, ctypes.byref(n), None))
+        keys = np.zeros(n.value, dtype=np.int32)
+        _check(lib().fmb_tree_level(self.h, l, ctypes.byref(n), _ptr(keys, ctypes.c_int)))
+        return keys
+
+    def num_sr(self, l):
+        num = np.zeros(len(self.level(l)), dtype=np.int32)
+        _check(lib().fmb_tree_num_sr(self.h, l, _ptr(num, ctypes.c_int)))
+        return num
+
+    def elem_box(self):
+        n = self.mesh.arrays()[1].shape[0]
+        box = np.zeros(n, dtype=np.int32)
+        perm = np.zeros(n, dtype=np.int32)
+        _check(lib().fmb_tree_elem_box(self.h, _ptr(box, ctypes.c_int), _ptr(perm, ctypes.c_int)))
+        return box, perm
+
+    def pairs(self, l):
+        n = ctypes.c_int()
+        _check(lib().fmb_tree_pairs(self.h, l, ctypes.byref(n), None, None, None))
+        t = np.zeros(n.value, dtype=np.int32)
+        s = np.zeros(n.value, dtype=np.int32)
+        o = np.zeros(n.value, dtype=np.int32)
+        _check(lib().fmb_tree_pairs(self.h, l, ctypes.byref(n), _ptr(t, ctypes.c_int),
+                                    _ptr(s, ctypes.c_int), _ptr(o, ctypes.c_int)))
+        return t, s, o
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fmb_tree_destroy(self.h)
+            self.h = None
+
+
+def plan_params(k, **kw):
+    p = PlanParams()
+    lib().fmb_plan_params_default(ctypes.byref(p))
+    p.k = k
+    for key, v in kw.items():
+        if key == "p_levels":
+            for i, q in enumerate(v):
+                p.p_levels[i] = int(q)
+        else:
+            setattr(p, key, v)
+    return p
+
+
+class Plan:
+    """Per-frequency FMM-BEM plan (fmb_plan)."""
+
+    def __init__(self, tree, k, robin=None, **kw):
+        self.tree = tree
+        self.params = plan_params(k, **kw)
+        self.h = _vp()
+        rb = _dev(robin) if robin is not None else None
+        _check(lib().fmb_plan_create(tree.h, ctypes.byref(self.params), rb, _stream(),
+                                     ctypes.byref(self.h)))
+
+    def orders(self):
+        p = np.zeros(16, dtype=np.int32)
+        _check(lib().fmb_plan_orders(self.h, _ptr(p, ctypes.c_int)))
+        return p
+
+    def memory(self):
+        b = ctypes.c_double()
+        _check(lib().fmb_plan_memory(self.h, ctypes.byref(b)))
+        return b.value
+
+    def matvec(self, x, y=None, prec=FMB_FP64):
+        import torch
+        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        if y is None:
+            y = torch.empty_like(x)
+        _check(lib().fmb_matvec(self.h, prec, _dev(x), _dev(y), nrhs, _stream()))
+        return y
+
+    def rhs_plane_wave(self, dirs):
+        import torch
+        dirs = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+        n = self.tree.mesh.arrays()[1].shape[0]
+        b = torch.empty((dirs.shape[0], n), dtype=torch.complex128, device="cuda")
+        _check(lib().fmb_rhs_plane_wave(self.h, _ptr(dirs, ctypes.c_double), dirs.shape[0],
+                                        _dev(b), _stream()))
+        return b
+
+    def gmres(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
+        import torch
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        info = GmresInfo()
+        rc = lib().fmb_gmres(self.h, _dev(b), _dev(x), nrhs, restart, tol, maxit,
+                             ctypes.byref(info), _stream())
+        if check:
+            _check(rc)
+        return x, info
+
+    def set_timing(self, on=True):
+        _check(lib().fmb_set_timing(self.h, int(on)))
+
+    def timing(self):
+        ph = np.zeros(7)
+        m2l = np.zeros(16)
+        _check(lib().fmb_last_timing(self.h, _ptr(ph, ctypes.c_double), _ptr(m2l, ctypes.c_double)))
+        return ph, m2l
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fmb_plan_destroy(self.h)
+            self.h = None
+
+
+def trunc_num(k, eps, sigma, a):
+    return lib().fmb_trunc_num(k, eps, sigma, a)
+
+
+def gen_rot_mats(angles, p):
+    angles = np.ascontiguousarray(np.atleast_2d(angles), dtype=np.float64)
+    n = angles.shape[0]
+    out = np.zeros((n, lib().fmb_rot_size(p)), dtype=np.complex128)
+    _check(lib().fmb_gen_rot_mats(_ptr(angles, ctypes.c_double), n, p,
+                                  out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def gen_coax_mats(kind, k, t, p):
+    t = np.ascontiguousarray(np.atleast_1d(t), dtype=np.float64)
+    out = np.zeros((t.shape[0], lib().fmb_coax_size(p)), dtype=np.complex128)
+    _check(lib().fmb_gen_coax_mats(kind, k, _ptr(t, ctypes.c_double), t.shape[0], p,
+                                   out.ctypes.data_as(ctypes.c_void_p)))
+    return out

@@ -1,0 +1,117 @@
+/*
+ * fmb_internal.h -- data structures shared by the C host code (csrc/host)
+ * and the HIP layer (csrc/hip).  Plain C: device buffers are void* or
+ * typed pointers into device memory.
+ */
+#ifndef FMB_INTERNAL_H
+#define FMB_INTERNAL_H
+
+#include <stddef.h>
+
+#include "../../include/fmbem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMB_MAXLEV 12 /* levels 0..11; the reference caps at 10 (octree.c:637) */
+#define FMB_NOFF 343  /* offset codes (dx+3)*49+(dy+3)*7+(dz+3)               */
+#define FMB_NDIST 15  /* distinct |offset|^2 among M2L offsets                */
+
+struct fmb_mesh {
+  int nv, nf;
+  double *V; /* nv*3 */
+  int *F;    /* nf*3 */
+};
+
+struct fmb_tree {
+  int nelem;
+  int lmin, lmax;
+  double D, pmin[3];
+  int nbox[FMB_MAXLEV];
+  /* device arrays */
+  int *d_keys[FMB_MAXLEV];   /* sorted Morton keys                       */
+  int *d_parent[FMB_MAXLEV]; /* l > 2: parent index in level l-1          */
+  int *d_cstart[FMB_MAXLEV]; /* l < lmax: children range in level l+1, nbox+1 */
+  int *d_perm;               /* sorted position -> caller element id       */
+  int *d_iperm;              /* caller element id -> sorted position       */
+  int *d_box_start;          /* leaf: nbox+1 element ranges (sorted order) */
+  int *d_elem_box;           /* sorted position -> leaf box                */
+  double *d_tri;             /* sorted order: 9 doubles per element        */
+  double *d_geo;             /* sorted order: centroid(3) normal(3) area radius */
+  int npairs[FMB_MAXLEV];
+  int *d_pair_tgt[FMB_MAXLEV], *d_pair_src[FMB_MAXLEV];
+  int *h_off_start[FMB_MAXLEV]; /* host: FMB_NOFF+1 bucket ranges           */
+  int *d_near_start, *d_near_box; /* leaf: near boxes incl. self (CSR)     */
+  int near_nnzbox;
+  int max_leaf_elems;
+  void *stream;
+};
+
+/* one precision's operators, bases, near field and workspace */
+typedef struct fmb_opset {
+  int prec;      /* FMB_FP64 or FMB_MIXED(=FP32 arithmetic)                 */
+  size_t csz;    /* bytes of one complex value                              */
+  void *d_H;     /* [nbeta][rot_size(Pmax)] real                            */
+  void *d_Csr[FMB_MAXLEV];  /* [15][coax_size(p_l)] complex, scaled           */
+  void *d_Cm2m[FMB_MAXLEV]; /* l>2: (R|R) child l -> parent l-1, scaled       */
+  void *d_Cl2l[FMB_MAXLEV]; /* l>2: (R|R) parent l-1 -> child l, scaled       */
+  void *d_P2M;  /* [N][pL^2] complex, sorted order, scaled                   */
+  void *d_L2P;  /* [N][pL^2] complex                                        */
+  void *d_near; /* near blocks, complex                                     */
+  void *d_M[FMB_MAXLEV], *d_L[FMB_MAXLEV]; /* [nbox][nrhs][p^2] complex      */
+  void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
+} fmb_opset;
+
+struct fmb_plan {
+  const fmb_tree *tree;
+  fmb_plan_params prm;
+  double k, alpha_re, alpha_im;
+  int p[FMB_MAXLEV];
+  int Pmax;
+  double *h_scale[FMB_MAXLEV]; /* w_n per level                              */
+  double *d_scale[FMB_MAXLEV];
+  int nbeta;
+  double beta_cos[FMB_NOFF];
+  int off_beta[FMB_NOFF];
+  double off_phi[FMB_NOFF];
+  int off_dist[FMB_NOFF];
+  /* octant o of a child: child centre - parent centre = (bits - 1/2) side */
+  int m2m_beta[8], l2l_beta[8]; /* t = parent - child, t = child - parent  */
+  double m2m_phi[8], l2l_phi[8];
+  int *d_m2m_beta, *d_l2l_beta;
+  double *d_m2m_phi, *d_l2l_phi;
+  void *d_off_info;     /* FMB_NOFF x {int beta, int dist, double phi}      */
+  void *d_m2l_chunks[FMB_MAXLEV]; /* int4 {offset, first, last, 0}          */
+  int n_m2l_chunks[FMB_MAXLEV];
+  double *d_quad;       /* far rule: nq x {b0,b1,b2,w}                    */
+  double *d_quadn;      /* close rule                                      */
+  double *d_edge;       /* nedge x {x, w} Gauss-Legendre on [-1,1]         */
+  int nq, nqn;
+  size_t *d_near_off;   /* per leaf box: offset of its block (elements)    */
+  size_t near_total;
+  int *d_near_ncol;     /* per leaf box: columns of its block              */
+  int near_max_ncol;
+  int *d_tab_mmajor[FMB_MAXLEV]; /* per level: m-major (n',m) enumeration   */
+  fmb_opset ops[2];     /* [0] FP64, [1] FP32 (FMB_MIXED only)            */
+  int nops;
+  int nrhs_max;
+  double *d_work64;     /* [nrhs][N] FP64 complex scratch                  */
+  int timing;
+  double t_phase[7];
+  double t_m2l[FMB_MAXLEV];
+  size_t bytes;
+  void *stream;
+};
+
+/* offset code helpers */
+static inline int fmb_off_code(int dx, int dy, int dz) {
+  return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3);
+}
+
+int fmb_set_error(int code, const char *fmt, ...);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

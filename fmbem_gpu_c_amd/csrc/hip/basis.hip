@@ -1,0 +1,232 @@
+// basis.hip -- per-element P2M and L2P bases, evaluated once per plan in FP64
+// and stored scaled in the compute precision.
+//
+//   P2M_j[n,m] = ik sum_q w_q A_j [ -dR_n^{-m}/dn_j + sigma_j R_n^{-m} ](y_q - c)
+//                / w_n                    (bottomLevelCoeff_L, integral.cu:821-836,
+//                                          triElemIntegral_R / _pRpn, 251-365)
+//   L2P_i[n,m] = [ R_n^m + alpha dR_n^m/dn_i ](x_i - c) / w_n
+// c is the leaf-box centre; dR/dn follows dirDev_R (integral.cu:284-331).
+// One wavefront per element: lane 0 runs the serial Bessel/Legendre
+// diagonals, the lanes share the Legendre columns and the R table in LDS.
+#include "fmb_device.h"
+
+namespace {
+
+constexpr int WG = 64;
+
+__device__ __forceinline__ int tri_idx(int n, int m) { return n * (n + 1) / 2 + m; }
+
+// Fill Rt[(n,m)] = R_n^m(v), n <= nm, in LDS.  Scratch in LDS as given.
+__device__ void r_table(int nm, double k, const double *v, double *jn,
+                        double *diag, double *Pt, double2 *ph, double2 *Rt,
+                        double *geo) {
+  int lane = threadIdx.x;
+  if (lane == 0) {
+    double rho2 = v[0] * v[0] + v[1] * v[1];
+    double r = sqrt(rho2 + v[2] * v[2]);
+    double ct = 1.0, st = 0.0, phi = 0.0;
+    if (r > 0) {
+      ct = v[2] / r;
+      st = sqrt(rho2) / r;
+      phi = atan2(v[1], v[0]);
+    }
+    geo[0] = ct;
+    geo[1] = st;
+    geo[2] = phi;
+    dev_sph_jn(nm, k * r, jn);
+    double d = 1.0 / sqrt(4.0 * M_PI);
+    diag[0] = d;
+    for (int m = 1; m <= nm; m++) {
+      d *= sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
+      diag[m] = d;
+    }
+  }
+  __syncthreads();
+  double ct = geo[0], phi = geo[2];
+  for (int m = lane; m <= nm; m += WG) {
+    double p2 = 0.0, p1 = diag[m];
+    Pt[tri_idx(m, m)] = p1;
+    for (int n = m + 1; n <= nm; n++) {
+      double pv;
+      if (n == m + 1)
+        pv = sqrt(2.0 * m + 3.0) * ct * p1;
+      else {
+        double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+        double b = sqrt((((double)(n - 1) * (n - 1)) - (double)m * m) /
+                        (4.0 * (n - 1) * (n - 1) - 1.0));
+        pv = a * (ct * p1 - b * p2);
+      }
+      Pt[tri_idx(n, m)] = pv;
+      p2 = p1;
+      p1 = pv;
+    }
+    double s, c;
+    sincos(m * phi, &s, &c);
+    ph[m] = make_double2(c, s);
+  }
+  __syncthreads();
+  for (int f = lane; f < (nm + 1) * (nm + 1); f += WG) {
+    int n = (int)sqrt((double)f);
+    while (n * n > f) n--;
+    while ((n + 1) * (n + 1) <= f) n++;
+    int m = f - n * n - n, am = m < 0 ? -m : m;
+    double a = jn[n] * Pt[tri_idx(n, am)];
+    double2 e = ph[am];
+    Rt[f] = make_double2(a * e.x, m < 0 ? -a * e.y : a * e.y);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double2 rget(const double2 *Rt, int n, int m) {
+  if (n < 0 || (m < 0 ? -m : m) > n) return make_double2(0.0, 0.0);
+  return Rt[n * n + n + m];
+}
+
+// dR_n^m/dn (integral.cu:284-331)
+__device__ __forceinline__ double2 dRdn(const double2 *Rt, int n, int m,
+                                        double k, const double *nr) {
+  double2 a1 = rget(Rt, n + 1, m + 1), a2 = rget(Rt, n - 1, m + 1);
+  double2 b1 = rget(Rt, n + 1, m - 1), b2 = rget(Rt, n - 1, m - 1);
+  double2 c1 = rget(Rt, n - 1, m), c2 = rget(Rt, n + 1, m);
+  double f0 = bcoef(n + 1, -m - 1), f1 = bcoef(n, m);
+  double f2 = bcoef(n + 1, m - 1), f3 = bcoef(n, -m);
+  double f4 = acoef(n - 1, m), f5 = acoef(n, m);
+  double t0r = f0 * a1.x - f1 * a2.x, t0i = f0 * a1.y - f1 * a2.y;
+  double t1r = f2 * b1.x - f3 * b2.x, t1i = f2 * b1.y - f3 * b2.y;
+  double t2r = f4 * c1.x - f5 * c2.x, t2i = f4 * c1.y - f5 * c2.y;
+  double hx = 0.5 * k * nr[0], hy = 0.5 * k * nr[1], kz = k * nr[2];
+  return make_double2(hx * t0r + hy * t0i + hx * t1r - hy * t1i + kz * t2r,
+                      hx * t0i - hy * t0r + hx * t1i + hy * t1r + kz * t2i);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(WG)
+    k_bases(int N, int pL, double k, double alre, double alim,
+            const double *__restrict__ tri, const double *__restrict__ geo,
+            const int *__restrict__ ebox, const int *__restrict__ keys, int L,
+            double D, double p0, double p1, double p2,
+            const double2 *__restrict__ robin, const double *__restrict__ quad,
+            int nq, const double *__restrict__ wsc, T *__restrict__ P2M,
+            T *__restrict__ L2P) {
+  extern __shared__ double sm[];
+  int nm = pL;  // R needed up to degree pL for the derivative of degree pL-1
+  int ntab = (nm + 1) * (nm + 2) / 2;
+  const int ps = pL * pL;
+  double2 *ph = (double2 *)sm;
+  double2 *Rt = ph + (nm + 1);
+  double2 *acc = Rt + (nm + 1) * (nm + 1);
+  double *jn = (double *)(acc + ps);
+  double *diag = jn + (nm + 1);
+  double *Pt = diag + (nm + 1);
+  double *g3 = Pt + ntab;
+  int j = blockIdx.x;
+  int lane = threadIdx.x;
+  // leaf-box centre
+  int key = keys[ebox[j]];
+  int tx = 0, ty = 0, tz = 0;
+  for (int i = 0; i < L; i++) {
+    int d = (key >> (3 * i)) & 7;
+    tx |= ((d >> 2) & 1) << i;
+    ty |= ((d >> 1) & 1) << i;
+    tz |= (d & 1) << i;
+  }
+  double h = ldexp(1.0, -L);
+  double c[3] = {D * (h * tx + 0.5 * h) + p0, D * (h * ty + 0.5 * h) + p1,
+                 D * (h * tz + 0.5 * h) + p2};
+  const double *tr = tri + 9 * (size_t)j;
+  const double *g = geo + 8 * (size_t)j;
+  double nr[3] = {g[3], g[4], g[5]};
+  double area = g[6];
+  double2 sg = robin ? robin[j] : make_double2(0.0, 0.0);
+  for (int f = lane; f < ps; f += WG) acc[f] = make_double2(0.0, 0.0);
+  for (int q = 0; q < nq; q++) {
+    double v[3];
+    const double *qq = quad + 4 * q;
+    for (int a = 0; a < 3; a++)
+      v[a] = qq[0] * tr[a] + qq[1] * tr[3 + a] + qq[2] * tr[6 + a] - c[a];
+    double w = qq[3] * area;
+    r_table(nm, k, v, jn, diag, Pt, ph, Rt, g3);
+    for (int f = lane; f < ps; f += WG) {
+      int n = (int)sqrt((double)f);
+      while (n * n > f) n--;
+      while ((n + 1) * (n + 1) <= f) n++;
+      int m = f - n * n - n;
+      double2 R = Rt[n * n + n - m];
+      double2 dR = dRdn(Rt, n, -m, k, nr);
+      // val = -dR + sigma R ; P2M += i k w val
+      double vr = -dR.x + sg.x * R.x - sg.y * R.y;
+      double vi = -dR.y + sg.x * R.y + sg.y * R.x;
+      acc[f].x += -k * w * vi;
+      acc[f].y += k * w * vr;
+    }
+    __syncthreads();
+  }
+  for (int f = lane; f < ps; f += WG) {
+    int n = (int)sqrt((double)f);
+    while (n * n > f) n--;
+    while ((n + 1) * (n + 1) <= f) n++;
+    double s = 1.0 / wsc[n];
+    P2M[2 * ((size_t)j * ps + f)] = (T)(acc[f].x * s);
+    P2M[2 * ((size_t)j * ps + f) + 1] = (T)(acc[f].y * s);
+  }
+  // L2P at the centroid
+  double v[3] = {g[0] - c[0], g[1] - c[1], g[2] - c[2]};
+  r_table(nm, k, v, jn, diag, Pt, ph, Rt, g3);
+  for (int f = lane; f < ps; f += WG) {
+    int n = (int)sqrt((double)f);
+    while (n * n > f) n--;
+    while ((n + 1) * (n + 1) <= f) n++;
+    int m = f - n * n - n;
+    double2 R = Rt[f];
+    double2 dR = dRdn(Rt, n, m, k, nr);
+    double er = R.x + alre * dR.x - alim * dR.y;
+    double ei = R.y + alre * dR.y + alim * dR.x;
+    double s = 1.0 / wsc[n];
+    L2P[2 * ((size_t)j * ps + f)] = (T)(er * s);
+    L2P[2 * ((size_t)j * ps + f) + 1] = (T)(ei * s);
+  }
+}
+
+}  // namespace
+
+extern "C" int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *robin,
+                          void *stream) {
+  const fmb_tree *t = pl->tree;
+  int L = t->lmax, pL = pl->p[L];
+  int nm = pL;
+  size_t lds = sizeof(double) * (2 * (nm + 1) + (nm + 1) * (nm + 2) / 2 + 4) +
+               sizeof(double2) * ((nm + 1) + (nm + 1) * (nm + 1) + pL * pL);
+  if (lds > 160 * 1024)
+    return fmb_set_error(FMB_ERR_RANGE, "leaf order %d too large for bases",
+                         pL);
+  hipStream_t st = (hipStream_t)stream;
+  fmb_opset *o = &pl->ops[op];
+  if (o->prec == FMB_FP64) {
+    auto kern = k_bases<double>;
+    if (lds > 64 * 1024)
+      FMB_HIP_CHECK(hipFuncSetAttribute(
+          (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+          (int)lds));
+    hipLaunchKernelGGL(kern, dim3(t->nelem), dim3(WG), lds, st, t->nelem, pL,
+                       pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
+                       t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],
+                       t->pmin[1], t->pmin[2], (const double2 *)robin,
+                       pl->d_quad, pl->nq, pl->d_scale[L], (double *)o->d_P2M,
+                       (double *)o->d_L2P);
+  } else {
+    auto kern = k_bases<float>;
+    if (lds > 64 * 1024)
+      FMB_HIP_CHECK(hipFuncSetAttribute(
+          (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+          (int)lds));
+    hipLaunchKernelGGL(kern, dim3(t->nelem), dim3(WG), lds, st, t->nelem, pL,
+                       pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
+                       t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],
+                       t->pmin[1], t->pmin[2], (const double2 *)robin,
+                       pl->d_quad, pl->nq, pl->d_scale[L], (float *)o->d_P2M,
+                       (float *)o->d_L2P);
+  }
+  FMB_LAUNCH_CHECK();
+  FMB_HIP_CHECK(hipStreamSynchronize(st));
+  return FMB_OK;
+}

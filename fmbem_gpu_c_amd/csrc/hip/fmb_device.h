@@ -1,0 +1,138 @@
+/*
+ * fmb_device.h -- device-side helpers shared by the gfx950 kernels:
+ * complex arithmetic on (re, im) pairs of the compute type, launch checks,
+ * FP64 special functions (spherical Bessel, normalised Legendre, R_n^m and
+ * its normal derivative) used by operator generation and the bases.
+ */
+#ifndef FMB_DEVICE_H
+#define FMB_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+
+#include "fmb_hip.h"
+
+#define FMB_HIP_CHECK(call)                                                    \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fmb_set_error(FMB_ERR_HIP, "%s:%d: %s", __FILE__, __LINE__,       \
+                           hipGetErrorString(e_));                             \
+  } while (0)
+
+#define FMB_LAUNCH_CHECK() FMB_HIP_CHECK(hipGetLastError())
+
+template <typename T> struct cplx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmk(T r, T i) {
+  cplx<T> z;
+  z.re = r;
+  z.im = i;
+  return z;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return cmk<T>(a.re + b.re, a.im + b.im);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cmk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+/* a += b*c */
+template <typename T>
+__device__ __forceinline__ void cfma(cplx<T> &a, cplx<T> b, cplx<T> c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
+}
+/* a += h*c, h real */
+template <typename T>
+__device__ __forceinline__ void rfma(cplx<T> &a, T h, cplx<T> c) {
+  a.re = fma(h, c.re, a.re);
+  a.im = fma(h, c.im, a.im);
+}
+
+static inline __host__ __device__ int rot_off(int n) {
+  return n * (4 * n * n - 1) / 3;
+}
+static inline __host__ __device__ int coax_size(int p) {
+  return p * (2 * p * p + 3 * p + 1) / 6;
+}
+static inline __host__ __device__ int coax_blk(int p, int m) {
+  int a = m < 0 ? -m : m;
+  return a * (2 * a * a - (3 + 6 * p) * a + 1 + 6 * (p + p * p)) / 6;
+}
+
+/* translation.cu:189-212 in FP64 */
+__device__ __forceinline__ double acoef(int n, int m) {
+  int am = m < 0 ? -m : m;
+  if (n < am) return 0.0;
+  double nf = n, mf = m;
+  return sqrt((nf + 1 + mf) * (nf + 1 - mf) / ((2 * nf + 1) * (2 * nf + 3)));
+}
+__device__ __forceinline__ double bcoef(int n, int m) {
+  double nf = n, mf = m;
+  if (m >= 0 && n >= m)
+    return sqrt((nf - mf - 1) * (nf - mf) / ((2 * nf - 1) * (2 * nf + 1)));
+  if (-n <= m && m < 0)
+    return -sqrt((nf - mf - 1) * (nf - mf) / ((2 * nf - 1) * (2 * nf + 1)));
+  return 0.0;
+}
+
+/* j_n(x), n = 0..nmax: Miller downward recurrence normalised by
+ * sum (2n+1) j_n^2 = 1; serial (one lane). */
+__device__ inline void dev_sph_jn(int nmax, double x, double *j) {
+  if (x == 0.0) {
+    j[0] = 1.0;
+    for (int n = 1; n <= nmax; n++) j[n] = 0.0;
+    return;
+  }
+  int nk = nmax < 1 ? 1 : nmax;
+  double ax = fabs(x);
+  double top = nk > ax ? (double)nk : ax;
+  int nst = (int)ceil(top) + 40 + (int)ceil(6.0 * cbrt(top));
+  double fp1 = 0.0, fc = 1.0, sum = 0.0, f0 = 0.0, f1 = 0.0;
+  for (int n = nst; n >= 0; n--) {
+    if (n <= nmax) j[n] = fc;
+    if (n == 0) f0 = fc;
+    if (n == 1) f1 = fc;
+    sum += (2.0 * n + 1.0) * fc * fc;
+    if (n == 0) break;
+    double fm1 = (2.0 * n + 1.0) / x * fc - fp1;
+    fp1 = fc;
+    fc = fm1;
+    if (fabs(fc) > 1e100) {
+      const double s = 1e-100;
+      fc *= s;
+      fp1 *= s;
+      sum *= s * s;
+      if (n <= 1) f1 *= s;
+      for (int q = (n - 1 > nmax ? nmax + 1 : n); q <= nmax; q++) j[q] *= s;
+    }
+  }
+  double c = 1.0 / sqrt(sum);
+  double sx, cx;
+  sincos(x, &sx, &cx);
+  double j0 = sx / x, j1 = sx / (x * x) - cx / x;
+  bool neg = (fabs(j0) >= fabs(j1)) ? ((j0 < 0) != (f0 < 0))
+                                    : ((j1 < 0) != (f1 < 0));
+  if (neg) c = -c;
+  for (int n = 0; n <= nmax; n++) j[n] *= c;
+}
+
+/* y_n(x) by upward recurrence */
+__device__ inline void dev_sph_yn(int nmax, double x, double *y) {
+  double sx, cx;
+  sincos(x, &sx, &cx);
+  y[0] = -cx / x;
+  if (nmax == 0) return;
+  y[1] = -cx / (x * x) - sx / x;
+  for (int n = 1; n < nmax; n++) y[n + 1] = (2.0 * n + 1.0) / x * y[n] - y[n - 1];
+}
+
+#endif

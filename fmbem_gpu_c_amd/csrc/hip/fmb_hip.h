@@ -1,0 +1,100 @@
+/*
+ * fmb_hip.h -- the thin C-ABI layer between the C host code and the HIP
+ * kernels.  Every function returns FMB_OK or a negative fmbem.h code.
+ * 'stream' is a hipStream_t passed as void*.
+ */
+#ifndef FMB_HIP_H
+#define FMB_HIP_H
+
+#include <stddef.h>
+
+#include "../fmb_internal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime */
+int fmbh_init(void);
+int fmbh_malloc(void **p, size_t bytes);
+int fmbh_free(void *p);
+int fmbh_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int fmbh_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int fmbh_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int fmbh_memset(void *dst, int v, size_t bytes, void *stream);
+int fmbh_sync(void *stream);
+int fmbh_event_create(void **ev);
+int fmbh_event_record(void *ev, void *stream);
+int fmbh_event_elapsed(void *ev0, void *ev1, float *ms);
+int fmbh_event_destroy(void *ev);
+
+/* ---- tree (tree.hip) */
+int fmbh_centroid_bbox(const double *d_V, const int *d_F, int n,
+                       double *d_ctr, double *h_lo3, double *h_hi3,
+                       void *stream);
+int fmbh_morton_sort(const double *d_ctr, int n, const double *pmin,
+                     double D, int *d_key10, int *d_perm, void *stream);
+int fmbh_deter_lmax(const int *d_key10, int n, int s, int *lmax,
+                    void *stream);
+int fmbh_leaf_level(fmb_tree *t, const int *d_key10, void *stream);
+int fmbh_parent_level(fmb_tree *t, int l, void *stream);
+int fmbh_m2l_pairs(fmb_tree *t, int l, void *stream);
+int fmbh_near_lists(fmb_tree *t, void *stream);
+int fmbh_sorted_geometry(fmb_tree *t, const double *d_V, const int *d_F,
+                         void *stream);
+
+/* ---- operators (ops.hip) */
+int fmbh_gen_H(const double *h_beta, int nbeta, int P, double *d_H,
+               void *stream);
+int fmbh_gen_coax(int kind, double k, const double *h_t, int n, int P,
+                  fmb_c64 *d_out, void *stream);
+int fmbh_coax_scale(int prec, const fmb_c64 *d_in, int P, const double *d_rs,
+                    const double *d_cs, int invert_rs, int invert_cs,
+                    void *d_out, void *stream);
+int fmbh_convert_real(int prec, const double *d_in, size_t n, void *d_out,
+                      void *stream);
+int fmbh_rot_phase(const double *d_H, int n, int P, const double *h_angles,
+                   fmb_c64 *d_out, void *stream);
+
+/* ---- bases and near field (basis.hip, near.hip) */
+int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *d_robin_sorted,
+               void *stream);
+int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *d_robin_sorted,
+                       void *stream);
+int fmbh_near_layout(fmb_plan *pl, void *stream);
+
+/* ---- FMM pass (fmm.hip) */
+int fmbh_gather(const fmb_plan *pl, int op, const fmb_c64 *x, int nrhs,
+                void *stream);
+int fmbh_scatter(const fmb_plan *pl, int op, fmb_c64 *y, int nrhs,
+                 void *stream);
+int fmbh_p2m(const fmb_plan *pl, int op, int nrhs, void *stream);
+int fmbh_m2m(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
+int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
+int fmbh_l2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
+int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, void *stream);
+int fmbh_mmajor_table(int p, int **d_tab, void *stream);
+
+/* ---- FP64 vector kernels for GMRES (blas.hip) */
+int fmbh_rhs_plane(const fmb_plan *pl, const double *h_dirs, int nrhs,
+                   fmb_c64 *d_b, void *stream);
+/* h[j*nrhs + r] = <V_j^r, w^r> for j < nv ; V: [nv][nrhs][N] */
+int fmbh_zdots(const fmb_c64 *d_V, int nv, const fmb_c64 *d_w, int nrhs,
+               int N, fmb_c64 *d_h, void *stream);
+/* w^r -= sum_j h[j*nrhs+r] V_j^r */
+int fmbh_zgemv_sub(const fmb_c64 *d_V, int nv, const fmb_c64 *d_h,
+                   fmb_c64 *d_w, int nrhs, int N, void *stream);
+/* nrm[r] = ||w^r|| */
+int fmbh_znorms(const fmb_c64 *d_w, int nrhs, int N, double *d_nrm,
+                void *stream);
+/* y^r = a[r] x^r + b[r] y^r  (a, b device complex per rhs; NULL b = 0) */
+int fmbh_zaxpby(const fmb_c64 *d_a, const fmb_c64 *d_x, const fmb_c64 *d_b,
+                fmb_c64 *d_y, int nrhs, int N, void *stream);
+/* x^r += sum_j y[j*nrhs+r] V_j^r  (y on device) */
+int fmbh_zgemv_add(const fmb_c64 *d_V, int nv, const fmb_c64 *d_y,
+                   fmb_c64 *d_x, int nrhs, int N, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

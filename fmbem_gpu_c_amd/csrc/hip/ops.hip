@@ -1,0 +1,327 @@
+// ops.hip -- FP64 generation of the rotate-coaxial-rotate operators on the
+// device with O(p^3) work and O(p^2) scratch per matrix, and their scaled
+// conversion to the compute precision.
+//
+//   rotation H_n^{m',m}(beta): initial column from the normalised Legendre
+//     functions (rotMatsInit, translation.cu:1064-1092), recurrence in m
+//     (sparseRotMatGen, translation.cu:1179-1204), symmetry for m < 0
+//     (translation.cu:1206-1219); phases e^{i m alpha} e^{-i m' gamma}
+//     (translation.cu:1229) are applied on the fly by the translation kernels.
+//   coaxial (R|R), (S|R): initial column (-1)^n' sqrt(2n'+1) f_n'(kt)
+//     (translation.cu:555-584, 919-949), sectorial recurrence
+//     (translation.cu:689-711), advancing in n (translation.cu:713-745),
+//     |m|-block storage (translation.cu:760-769).
+// One workgroup per matrix (rotation, sectorial pass) or per |m| block
+// (advancing pass); the recurrences run sequentially in m (resp. n) with the
+// orthogonal index spread over the workgroup.
+#include "fmb_device.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+__global__ void __launch_bounds__(NT)
+    k_gen_H(const double *betas, int P, double *H, double *scratch) {
+  int b = blockIdx.x;
+  double beta = betas[b];
+  int nmax = 2 * P - 2;
+  int sz = (nmax + 1) * (nmax + 1);
+  double *col = scratch + (size_t)b * 2 * sz;
+  double *nxt = col + sz;
+  double *Hb = H + (size_t)b * rot_off(P);
+  double cb, sb;
+  sincos(beta, &sb, &cb);
+  sb = fabs(sb);
+  __shared__ double diag[512];
+  if (threadIdx.x == 0) {
+    double v = 1.0 / sqrt(4.0 * M_PI);
+    diag[0] = v;
+    for (int m = 1; m <= nmax; m++) {
+      v *= sqrt((2.0 * m + 1.0) / (2.0 * m)) * sb;
+      diag[m] = v;
+    }
+  }
+  __syncthreads();
+  // initial column H_n^{m',0} = sqrt(4pi/(2n+1)) P~_n^{|m'|}(cos beta)
+  for (int m = threadIdx.x; m <= nmax; m += NT) {
+    double p2 = 0.0, p1 = diag[m];
+    for (int n = m; n <= nmax; n++) {
+      double pv;
+      if (n == m)
+        pv = p1;
+      else if (n == m + 1)
+        pv = sqrt(2.0 * m + 3.0) * cb * p1;
+      else {
+        double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+        double bq = sqrt((((double)(n - 1) * (n - 1)) - (double)m * m) /
+                         (4.0 * (n - 1) * (n - 1) - 1.0));
+        pv = a * (cb * p1 - bq * p2);
+      }
+      if (n > m) {
+        p2 = p1;
+        p1 = pv;
+      }
+      double hv = sqrt(4.0 * M_PI / (2.0 * n + 1.0)) * pv;
+      col[n * n + n + m] = hv;
+      col[n * n + n - m] = hv;
+    }
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < P * P; f += NT) {
+    int n = (int)sqrt((double)f);
+    while (n * n > f) n--;
+    while ((n + 1) * (n + 1) <= f) n++;
+    int mp = f - n * n - n;
+    Hb[rot_off(n) + n * (2 * n + 1) + (mp + n)] = col[f];
+  }
+  for (int m = 0; m + 1 <= P - 1; m++) {
+    int nlo = m + 1, nhi = nmax - (m + 1);
+    int g0 = nlo * nlo, g1 = (nhi + 1) * (nhi + 1);
+    double bd_unused = 0.0;
+    (void)bd_unused;
+    for (int g = g0 + threadIdx.x; g < g1; g += NT) {
+      int n = (int)sqrt((double)g);
+      while (n * n > g) n--;
+      while ((n + 1) * (n + 1) <= g) n++;
+      int mp = g - n * n - n;
+      double t0 = 0.5 * bcoef(n + 1, -mp - 1) * (1.0 - cb);
+      double t1 = 0.5 * bcoef(n + 1, mp - 1) * (1.0 + cb);
+      double t2 = acoef(n, mp) * sb;
+      int b1 = (n + 1) * (n + 1) + (n + 1);
+      double v = t0 * col[b1 + mp + 1] - t1 * col[b1 + mp - 1] -
+                 t2 * col[b1 + mp];
+      nxt[g] = v / bcoef(n + 1, m);
+    }
+    __syncthreads();
+    double *tmp = col;
+    col = nxt;
+    nxt = tmp;
+    int q0 = (m + 1) * (m + 1);
+    for (int g = q0 + threadIdx.x; g < P * P; g += NT) {
+      int n = (int)sqrt((double)g);
+      while (n * n > g) n--;
+      while ((n + 1) * (n + 1) <= g) n++;
+      int mp = g - n * n - n;
+      Hb[rot_off(n) + (m + 1 + n) * (2 * n + 1) + (mp + n)] = col[g];
+    }
+    __syncthreads();
+  }
+  // m < 0: H_n^{m',m} = H_n^{-m',-m}
+  for (int g = threadIdx.x; g < P * P; g += NT) {
+    int n = (int)sqrt((double)g);
+    while (n * n > g) n--;
+    while ((n + 1) * (n + 1) <= g) n++;
+    int mp = g - n * n - n;
+    int ld = 2 * n + 1, off = rot_off(n);
+    for (int m = -n; m < 0; m++)
+      Hb[off + (m + n) * ld + (mp + n)] = Hb[off + (-m + n) * ld + (-mp + n)];
+  }
+}
+
+// sectorial values S[m][n'] = E(m, n', m), n' in [m, 2P-2-m]
+__global__ void __launch_bounds__(NT)
+    k_coax_sect(int kind, double k, const double *ts, int P, double2 *S) {
+  int d = blockIdx.x;
+  int nmax = 2 * P - 2, ld = 2 * P - 1;
+  double2 *Sd = S + (size_t)d * P * ld;
+  __shared__ double jn[512], yn[512];
+  if (threadIdx.x == 0) {
+    double x = k * ts[d];
+    dev_sph_jn(nmax, x, jn);
+    if (kind == FMB_COAX_SR) dev_sph_yn(nmax, x, yn);
+  }
+  __syncthreads();
+  for (int np = threadIdx.x; np <= nmax; np += NT) {
+    double s = ((np & 1) ? -1.0 : 1.0) * sqrt(2.0 * np + 1.0);
+    Sd[np] = make_double2(s * jn[np], kind == FMB_COAX_SR ? s * yn[np] : 0.0);
+  }
+  __syncthreads();
+  for (int m = 0; m + 1 <= P - 1; m++) {
+    double den = bcoef(m + 1, -m - 1);
+    const double2 *cur = Sd + (size_t)m * ld;
+    double2 *nx = Sd + (size_t)(m + 1) * ld;
+    for (int np = m + 1 + threadIdx.x; np <= nmax - (m + 1); np += NT) {
+      double a0 = bcoef(np, -m - 1) / den, a1 = bcoef(np + 1, m) / den;
+      double2 u = cur[np - 1], v = cur[np + 1];
+      nx[np] = make_double2(a0 * u.x - a1 * v.x, a0 * u.y - a1 * v.y);
+    }
+    __syncthreads();
+  }
+}
+
+// advancing in n inside block m; writes the |m| block of the sparse matrix
+__global__ void __launch_bounds__(NT)
+    k_coax_adv(int P, const double2 *S, double2 *out) {
+  int d = blockIdx.y, m = blockIdx.x;
+  int nmax = 2 * P - 2, ld = 2 * P - 1;
+  const double2 *Sm = S + ((size_t)d * P + m) * ld;
+  double2 *blk = out + (size_t)d * coax_size(P) + coax_blk(P, m);
+  int bl = P - m;
+  extern __shared__ double2 sh[];
+  double2 *prev = sh, *cur = sh + ld, *nxt = sh + 2 * ld;
+  for (int np = threadIdx.x; np < ld; np += NT) {
+    prev[np] = make_double2(0.0, 0.0);
+    cur[np] = (np >= m && np <= nmax - m) ? Sm[np] : make_double2(0.0, 0.0);
+    nxt[np] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int np = m + threadIdx.x; np < P; np += NT)
+    blk[(size_t)(m - m) * bl + (np - m)] = cur[np];
+  for (int n = m; n + 1 <= P - 1; n++) {
+    double an = acoef(n, m);
+    for (int np = m + threadIdx.x; np <= nmax - (n + 1); np += NT) {
+      double2 v0 = (np - 1 >= m) ? cur[np - 1] : make_double2(0.0, 0.0);
+      double2 v1 = cur[np + 1];
+      double2 v2 = (n - 1 >= m) ? prev[np] : make_double2(0.0, 0.0);
+      double c0 = acoef(np - 1, m), c1 = acoef(np, m), c2 = acoef(n - 1, m);
+      nxt[np] = make_double2((c0 * v0.x - c1 * v1.x + c2 * v2.x) / an,
+                             (c0 * v0.y - c1 * v1.y + c2 * v2.y) / an);
+    }
+    __syncthreads();
+    double2 *t = prev;
+    prev = cur;
+    cur = nxt;
+    nxt = t;
+    for (int np = m + threadIdx.x; np < P; np += NT)
+      blk[(size_t)(n + 1 - m) * bl + (np - m)] = cur[np];
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void k_coax_scale(const double2 *in, int P, const double *rs,
+                             const double *cs, int inv_r, int inv_c, T *out) {
+  int m = blockIdx.x, bl = P - m;
+  size_t off = coax_blk(P, m);
+  for (int e = threadIdx.x; e < bl * bl; e += blockDim.x) {
+    int np = m + e % bl, n = m + e / bl;
+    double r = rs ? (inv_r ? 1.0 / rs[np] : rs[np]) : 1.0;
+    double c = cs ? (inv_c ? 1.0 / cs[n] : cs[n]) : 1.0;
+    double2 v = in[off + e];
+    out[2 * (off + e)] = (T)(r * c * v.x);
+    out[2 * (off + e) + 1] = (T)(r * c * v.y);
+  }
+}
+
+template <typename T>
+__global__ void k_convert(const double *in, size_t n, T *out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) out[i] = (T)in[i];
+}
+
+__global__ void k_rot_phase(const double *H, int P, const double *ang,
+                            double2 *out) {
+  int a = blockIdx.y;
+  double al = ang[3 * a], ga = ang[3 * a + 2];
+  const double *Hb = H + (size_t)a * rot_off(P);
+  double2 *o = out + (size_t)a * rot_off(P);
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < P * P * P;
+       f += gridDim.x * blockDim.x) {
+    // enumerate (n, m', m) over the sparse layout by flat sparse index
+    if (f >= rot_off(P)) break;
+    int n = 0;
+    while (rot_off(n + 1) <= f) n++;
+    int e = f - rot_off(n), ld = 2 * n + 1;
+    int mp = e % ld - n, m = e / ld - n;
+    double s, c;
+    sincos(m * al - mp * ga, &s, &c);
+    o[f] = make_double2(Hb[f] * c, Hb[f] * s);
+  }
+}
+
+inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" int fmbh_gen_H(const double *h_beta, int nbeta, int P, double *d_H,
+                          void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (P < 1 || 2 * P - 2 >= 512)
+    return fmb_set_error(FMB_ERR_RANGE, "rotation order %d out of range", P);
+  double *d_b, *scr;
+  size_t sz = (size_t)(2 * P - 1) * (2 * P - 1);
+  int rc = fmbh_malloc((void **)&d_b, sizeof(double) * nbeta);
+  if (!rc) rc = fmbh_malloc((void **)&scr, sizeof(double) * 2 * sz * nbeta);
+  if (rc) return rc;
+  if ((rc = fmbh_h2d(d_b, h_beta, sizeof(double) * nbeta, stream))) return rc;
+  hipLaunchKernelGGL(k_gen_H, dim3(nbeta), dim3(NT), 0, st, d_b, P, d_H, scr);
+  FMB_LAUNCH_CHECK();
+  FMB_HIP_CHECK(hipStreamSynchronize(st));
+  fmbh_free(d_b);
+  fmbh_free(scr);
+  return FMB_OK;
+}
+
+extern "C" int fmbh_gen_coax(int kind, double k, const double *h_t, int n,
+                             int P, fmb_c64 *d_out, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (P < 1 || 2 * P - 2 >= 512)
+    return fmb_set_error(FMB_ERR_RANGE, "coaxial order %d out of range", P);
+  double *d_t;
+  double2 *S;
+  int ld = 2 * P - 1;
+  int rc = fmbh_malloc((void **)&d_t, sizeof(double) * n);
+  if (!rc) rc = fmbh_malloc((void **)&S, sizeof(double2) * (size_t)n * P * ld);
+  if (rc) return rc;
+  if ((rc = fmbh_h2d(d_t, h_t, sizeof(double) * n, stream))) return rc;
+  hipLaunchKernelGGL(k_coax_sect, dim3(n), dim3(NT), 0, st, kind, k, d_t, P, S);
+  FMB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_coax_adv, dim3(P, n), dim3(NT),
+                     3 * ld * sizeof(double2), st, P, S, (double2 *)d_out);
+  FMB_LAUNCH_CHECK();
+  FMB_HIP_CHECK(hipStreamSynchronize(st));
+  fmbh_free(d_t);
+  fmbh_free(S);
+  return FMB_OK;
+}
+
+extern "C" int fmbh_coax_scale(int prec, const fmb_c64 *d_in, int P,
+                               const double *d_rs, const double *d_cs,
+                               int invert_rs, int invert_cs, void *d_out,
+                               void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (prec == FMB_FP64)
+    hipLaunchKernelGGL(k_coax_scale<double>, dim3(P), dim3(256), 0, st,
+                       (const double2 *)d_in, P, d_rs, d_cs, invert_rs,
+                       invert_cs, (double *)d_out);
+  else
+    hipLaunchKernelGGL(k_coax_scale<float>, dim3(P), dim3(256), 0, st,
+                       (const double2 *)d_in, P, d_rs, d_cs, invert_rs,
+                       invert_cs, (float *)d_out);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+extern "C" int fmbh_convert_real(int prec, const double *d_in, size_t n,
+                                 void *d_out, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  int nb = nblk(n);
+  if (nb > 8192) nb = 8192;
+  if (nb < 1) nb = 1;
+  if (prec == FMB_FP64)
+    hipLaunchKernelGGL(k_convert<double>, dim3(nb), dim3(256), 0, st, d_in, n,
+                       (double *)d_out);
+  else
+    hipLaunchKernelGGL(k_convert<float>, dim3(nb), dim3(256), 0, st, d_in, n,
+                       (float *)d_out);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+extern "C" int fmbh_rot_phase(const double *d_H, int n, int P,
+                              const double *h_angles, fmb_c64 *d_out,
+                              void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  double *d_a;
+  int rc = fmbh_malloc((void **)&d_a, sizeof(double) * 3 * n);
+  if (rc) return rc;
+  if ((rc = fmbh_h2d(d_a, h_angles, sizeof(double) * 3 * n, stream))) return rc;
+  int nb = nblk(rot_off(P));
+  hipLaunchKernelGGL(k_rot_phase, dim3(nb, n), dim3(256), 0, st, d_H, P, d_a,
+                     (double2 *)d_out);
+  FMB_LAUNCH_CHECK();
+  FMB_HIP_CHECK(hipStreamSynchronize(st));
+  fmbh_free(d_a);
+  return FMB_OK;
+}

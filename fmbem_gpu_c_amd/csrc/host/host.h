@@ -1,0 +1,8 @@
+/* host.h -- helpers shared by the C host modules. */
+#ifndef FMB_HOST_H
+#define FMB_HOST_H
+#include "../fmb_internal.h"
+void fmbx_gauss_legendre(int n, double *x, double *w);
+int fmbx_tri_rule(int rule, int order, double **out);
+void fmbx_abs_hn(int nmax, double x, double *out);
+#endif

@@ -1,0 +1,496 @@
+/* plan.c -- per-frequency plan (the operator half of genOctree,
+ * translation.cu:3078-3348, plus bases and near field) and the FMM pass
+ * (OprL, translation.cu:3412-3769, completed with L2P and near field). */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../fmb_internal.h"
+#include "../hip/fmb_hip.h"
+#include "host.h"
+
+void fmb_plan_params_default(fmb_plan_params *p) {
+  memset(p, 0, sizeof(*p));
+  p->precision = FMB_FP64;
+  p->p_rule = FMB_PRULE_TRUNCNUM;
+  p->eps = 0.05;   /* initOctree eps (translation.cu:2991) */
+  p->sigma = 1.5;  /* truncNum sigma at every call (translation.cu:3432) */
+  p->p_min = 1;
+  p->quad_rule = FMB_RULE_GAUSS7;
+  p->quad_order = 3;
+  p->near_rule = FMB_RULE_COLLAPSED;
+  p->near_order = 8;
+  p->near_ratio = 2.0;
+  p->nedge = 24;
+  p->nrhs_max = 1;
+}
+
+static const int k_dist2[FMB_NDIST] = {4, 5, 6, 8, 9, 10, 11, 12,
+                                       13, 14, 17, 18, 19, 22, 27};
+
+static int dist_class(int d2) {
+  for (int i = 0; i < FMB_NDIST; i++)
+    if (k_dist2[i] == d2) return i;
+  return -1;
+}
+
+static int beta_of(fmb_plan *pl, double cb) {
+  for (int i = 0; i < pl->nbeta; i++)
+    if (fabs(pl->beta_cos[i] - cb) < 1e-12) return i;
+  pl->beta_cos[pl->nbeta] = cb;
+  return pl->nbeta++;
+}
+
+static int order_for(const fmb_plan_params *prm, double k, double side, int l) {
+  int p;
+  if (prm->p_rule == FMB_PRULE_EXPLICIT)
+    p = prm->p_levels[l];
+  else if (prm->p_rule == FMB_PRULE_EXCESS) {
+    double ka = k * sqrt(3.0) * side;
+    p = (int)ceil(ka + 1.8 * log10(ka + M_PI));
+  } else
+    p = fmb_trunc_num(k, prm->eps, prm->sigma, side);
+  if (p < prm->p_min) p = prm->p_min;
+  return p;
+}
+
+static int opset_alloc(fmb_plan *pl, fmb_opset *o, int prec) {
+  const fmb_tree *t = pl->tree;
+  int rc, L = t->lmax, N = t->nelem, nr = pl->nrhs_max;
+  o->prec = prec;
+  o->csz = prec == FMB_FP64 ? 16 : 8;
+  size_t rsz = o->csz / 2;
+  size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
+  if ((rc = fmbh_malloc(&o->d_H, rsz * hs))) return rc;
+  pl->bytes += rsz * hs;
+  for (int l = 2; l <= L; l++) {
+    size_t cs = (size_t)fmb_coax_size(pl->p[l]) * FMB_NDIST * o->csz;
+    if ((rc = fmbh_malloc(&o->d_Csr[l], cs))) return rc;
+    pl->bytes += cs;
+    if (l > 2) {
+      int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
+      size_t c2 = (size_t)fmb_coax_size(P) * o->csz;
+      if ((rc = fmbh_malloc(&o->d_Cm2m[l], c2))) return rc;
+      if ((rc = fmbh_malloc(&o->d_Cl2l[l], c2))) return rc;
+      pl->bytes += 2 * c2;
+    }
+    size_t ex = (size_t)t->nbox[l] * nr * pl->p[l] * pl->p[l] * o->csz;
+    if ((rc = fmbh_malloc(&o->d_M[l], ex))) return rc;
+    if ((rc = fmbh_malloc(&o->d_L[l], ex))) return rc;
+    pl->bytes += 2 * ex;
+  }
+  size_t bz = (size_t)N * pl->p[L] * pl->p[L] * o->csz;
+  if ((rc = fmbh_malloc(&o->d_P2M, bz))) return rc;
+  if ((rc = fmbh_malloc(&o->d_L2P, bz))) return rc;
+  pl->bytes += 2 * bz;
+  size_t nz = pl->near_total * o->csz;
+  if ((rc = fmbh_malloc(&o->d_near, nz))) return rc;
+  pl->bytes += nz;
+  size_t vz = (size_t)N * nr * o->csz;
+  if ((rc = fmbh_malloc(&o->d_xs, vz))) return rc;
+  if ((rc = fmbh_malloc(&o->d_ys, vz))) return rc;
+  pl->bytes += 2 * vz;
+  return FMB_OK;
+}
+
+static void opset_free(fmb_opset *o) {
+  fmbh_free(o->d_H);
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    fmbh_free(o->d_Csr[l]);
+    fmbh_free(o->d_Cm2m[l]);
+    fmbh_free(o->d_Cl2l[l]);
+    fmbh_free(o->d_M[l]);
+    fmbh_free(o->d_L[l]);
+  }
+  fmbh_free(o->d_P2M);
+  fmbh_free(o->d_L2P);
+  fmbh_free(o->d_near);
+  fmbh_free(o->d_xs);
+  fmbh_free(o->d_ys);
+  memset(o, 0, sizeof(*o));
+}
+
+void fmb_plan_destroy(fmb_plan *pl) {
+  if (!pl) return;
+  for (int i = 0; i < 2; i++) opset_free(&pl->ops[i]);
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    free(pl->h_scale[l]);
+    fmbh_free(pl->d_scale[l]);
+    fmbh_free(pl->d_m2l_chunks[l]);
+    fmbh_free(pl->d_tab_mmajor[l]);
+  }
+  fmbh_free(pl->d_m2m_beta);
+  fmbh_free(pl->d_l2l_beta);
+  fmbh_free(pl->d_m2m_phi);
+  fmbh_free(pl->d_l2l_phi);
+  fmbh_free(pl->d_off_info);
+  fmbh_free(pl->d_quad);
+  fmbh_free(pl->d_quadn);
+  fmbh_free(pl->d_edge);
+  fmbh_free(pl->d_near_off);
+  fmbh_free(pl->d_near_ncol);
+  fmbh_free(pl->d_work64);
+  free(pl);
+}
+
+typedef struct {
+  int beta, dist;
+  double phi;
+} off_info_t;
+
+/* operators of one precision from the FP64 originals */
+static int build_opset(fmb_plan *pl, fmb_opset *o, const double *d_H64,
+                       fmb_c64 *const *d_sr64, fmb_c64 *const *d_rr64,
+                       const fmb_c64 *d_robin) {
+  const fmb_tree *t = pl->tree;
+  int rc, L = t->lmax;
+  void *st = pl->stream;
+  size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
+  if ((rc = fmbh_convert_real(o->prec, d_H64, hs, o->d_H, st))) return rc;
+  for (int l = 2; l <= L; l++) {
+    int p = pl->p[l];
+    size_t cs = (size_t)fmb_coax_size(p);
+    for (int d = 0; d < FMB_NDIST; d++) {
+      /* M2L: L~ = w L, M = w M~  ->  diag(w) (S|R) diag(w) */
+      if ((rc = fmbh_coax_scale(o->prec, d_sr64[l] + cs * d, p, pl->d_scale[l],
+                                pl->d_scale[l], 0, 0,
+                                (char *)o->d_Csr[l] + cs * d * o->csz, st)))
+        return rc;
+    }
+    if (l > 2) {
+      int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
+      /* M2M child l -> parent l-1: diag(1/w_{l-1}) (R|R) diag(w_l) */
+      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l - 1],
+                                pl->d_scale[l], 1, 0, o->d_Cm2m[l], st)))
+        return rc;
+      /* L2L parent l-1 -> child l: diag(w_l) (R|R) diag(1/w_{l-1}) */
+      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l],
+                                pl->d_scale[l - 1], 0, 1, o->d_Cl2l[l], st)))
+        return rc;
+    }
+  }
+  if ((rc = fmbh_bases(pl, (int)(o - pl->ops), d_robin, st))) return rc;
+  if ((rc = fmbh_near_assemble(pl, (int)(o - pl->ops), d_robin, st))) return rc;
+  return fmbh_sync(st);
+}
+
+int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
+                    const fmb_c64 *robin, void *stream, fmb_plan **out) {
+  if (!t || !prm || !out || !(prm->k > 0) || prm->nrhs_max < 1 ||
+      (prm->precision != FMB_FP64 && prm->precision != FMB_MIXED))
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: invalid parameters");
+  if (prm->nedge < 1 || prm->nedge > 256 || prm->near_ratio < 0)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: quadrature parameters");
+  int rc;
+  fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
+  pl->tree = t;
+  pl->prm = *prm;
+  pl->stream = stream;
+  pl->k = prm->k;
+  if (prm->alpha_re == 0.0 && prm->alpha_im == 0.0) {
+    pl->alpha_re = 0.0;
+    pl->alpha_im = 1.0 / prm->k; /* alpha = i/k */
+  } else {
+    pl->alpha_re = prm->alpha_re;
+    pl->alpha_im = prm->alpha_im;
+  }
+  pl->nrhs_max = prm->nrhs_max;
+  int L = t->lmax;
+  fmb_c64 *d_robin = NULL;
+  double *d_H64 = NULL;
+  fmb_c64 *d_sr64[FMB_MAXLEV] = {0}, *d_rr64[FMB_MAXLEV] = {0};
+  /* orders */
+  pl->Pmax = 0;
+  for (int l = 2; l <= L; l++) {
+    double side = t->D * ldexp(1.0, -l);
+    pl->p[l] = order_for(prm, pl->k, side, l);
+    if (pl->p[l] < 1 || pl->p[l] > 200) {
+      rc = fmb_set_error(FMB_ERR_RANGE, "order %d at level %d", pl->p[l], l);
+      goto fail;
+    }
+    if (pl->p[l] > pl->Pmax) pl->Pmax = pl->p[l];
+  }
+  /* scale factors w_n = min(1, 1/(x |h_n(x)|)), x = k * box radius */
+  for (int l = 2; l <= L; l++) {
+    double x = pl->k * t->D * ldexp(1.0, -l) * sqrt(3.0) / 2.0;
+    int n1 = pl->Pmax + 1;
+    pl->h_scale[l] = (double *)malloc(sizeof(double) * n1);
+    fmbx_abs_hn(n1 - 1, x, pl->h_scale[l]);
+    for (int n = 0; n < n1; n++) {
+      double w = 1.0 / (x * pl->h_scale[l][n]);
+      if (!(w > 1e-280)) w = 1e-280;
+      pl->h_scale[l][n] = w < 1.0 ? w : 1.0;
+    }
+    if ((rc = fmbh_malloc((void **)&pl->d_scale[l], sizeof(double) * n1))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_scale[l], pl->h_scale[l], sizeof(double) * n1, stream)))
+      goto fail;
+  }
+  /* directions: M2L offsets and the child/parent diagonals */
+  pl->nbeta = 0;
+  for (int o = 0; o < FMB_NOFF; o++) {
+    int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
+    int d2 = dx * dx + dy * dy + dz * dz;
+    pl->off_beta[o] = pl->off_dist[o] = -1;
+    pl->off_phi[o] = 0.0;
+    if (dist_class(d2) < 0 || (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1))
+      continue;
+    pl->off_beta[o] = beta_of(pl, dz / sqrt((double)d2));
+    pl->off_dist[o] = dist_class(d2);
+    pl->off_phi[o] = atan2((double)dy, (double)dx);
+  }
+  for (int oct = 0; oct < 8; oct++) {
+    double d[3] = {((oct >> 2) & 1) - 0.5, ((oct >> 1) & 1) - 0.5, (oct & 1) - 0.5};
+    double r = sqrt(0.75);
+    pl->l2l_beta[oct] = beta_of(pl, d[2] / r);
+    pl->l2l_phi[oct] = atan2(d[1], d[0]);
+    pl->m2m_beta[oct] = beta_of(pl, -d[2] / r);
+    pl->m2m_phi[oct] = atan2(-d[1], -d[0]);
+  }
+  {
+    off_info_t info[FMB_NOFF];
+    for (int o = 0; o < FMB_NOFF; o++) {
+      info[o].beta = pl->off_beta[o] < 0 ? 0 : pl->off_beta[o];
+      info[o].dist = pl->off_dist[o] < 0 ? 0 : pl->off_dist[o];
+      info[o].phi = pl->off_phi[o];
+    }
+    if ((rc = fmbh_malloc(&pl->d_off_info, sizeof(info)))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_off_info, info, sizeof(info), stream))) goto fail;
+    if ((rc = fmbh_malloc((void **)&pl->d_m2m_beta, sizeof(int) * 8))) goto fail;
+    if ((rc = fmbh_malloc((void **)&pl->d_l2l_beta, sizeof(int) * 8))) goto fail;
+    if ((rc = fmbh_malloc((void **)&pl->d_m2m_phi, sizeof(double) * 8))) goto fail;
+    if ((rc = fmbh_malloc((void **)&pl->d_l2l_phi, sizeof(double) * 8))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_m2m_beta, pl->m2m_beta, sizeof(int) * 8, stream))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_l2l_beta, pl->l2l_beta, sizeof(int) * 8, stream))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_m2m_phi, pl->m2m_phi, sizeof(double) * 8, stream))) goto fail;
+    if ((rc = fmbh_h2d(pl->d_l2l_phi, pl->l2l_phi, sizeof(double) * 8, stream))) goto fail;
+  }
+  /* M2L work chunks per level: pairs of one offset bucket */
+  for (int l = 2; l <= L; l++) {
+    int np = t->npairs[l];
+    int ch = np / 4096;
+    if (ch < 1) ch = 1;
+    if (ch > 16) ch = 16;
+    int cap = np / ch + FMB_NOFF + 1, n = 0;
+    int *c = (int *)malloc(sizeof(int) * 4 * cap);
+    for (int o = 0; o < FMB_NOFF && np > 0; o++)
+      for (int s = t->h_off_start[l][o]; s < t->h_off_start[l][o + 1]; s += ch) {
+        int e = s + ch < t->h_off_start[l][o + 1] ? s + ch : t->h_off_start[l][o + 1];
+        c[4 * n] = o;
+        c[4 * n + 1] = s;
+        c[4 * n + 2] = e;
+        c[4 * n + 3] = 0;
+        n++;
+      }
+    pl->n_m2l_chunks[l] = n;
+    rc = fmbh_malloc(&pl->d_m2l_chunks[l], sizeof(int) * 4 * (n > 0 ? n : 1));
+    if (!rc && n) rc = fmbh_h2d(pl->d_m2l_chunks[l], c, sizeof(int) * 4 * n, stream);
+    free(c);
+    if (rc) goto fail;
+    if ((rc = fmbh_mmajor_table(pl->p[l], &pl->d_tab_mmajor[l], stream))) goto fail;
+  }
+  /* quadrature tables */
+  {
+    double *q = NULL;
+    int nq = fmbx_tri_rule(prm->quad_rule, prm->quad_order, &q);
+    if (nq < 1) {
+      rc = fmb_set_error(FMB_ERR_ARG, "panel rule %d order %d", prm->quad_rule,
+                         prm->quad_order);
+      goto fail;
+    }
+    pl->nq = nq;
+    rc = fmbh_malloc((void **)&pl->d_quad, sizeof(double) * 4 * nq);
+    if (!rc) rc = fmbh_h2d(pl->d_quad, q, sizeof(double) * 4 * nq, stream);
+    free(q);
+    if (rc) goto fail;
+    int nqn = fmbx_tri_rule(prm->near_rule, prm->near_order, &q);
+    if (nqn < 1) {
+      rc = fmb_set_error(FMB_ERR_ARG, "close-pair rule %d order %d",
+                         prm->near_rule, prm->near_order);
+      goto fail;
+    }
+    pl->nqn = nqn;
+    rc = fmbh_malloc((void **)&pl->d_quadn, sizeof(double) * 4 * nqn);
+    if (!rc) rc = fmbh_h2d(pl->d_quadn, q, sizeof(double) * 4 * nqn, stream);
+    free(q);
+    if (rc) goto fail;
+    int ne = prm->nedge;
+    double *gx = (double *)malloc(sizeof(double) * ne),
+           *gw = (double *)malloc(sizeof(double) * ne),
+           *e = (double *)malloc(sizeof(double) * 2 * ne);
+    fmbx_gauss_legendre(ne, gx, gw);
+    for (int i = 0; i < ne; i++) {
+      e[2 * i] = gx[i];
+      e[2 * i + 1] = gw[i];
+    }
+    rc = fmbh_malloc((void **)&pl->d_edge, sizeof(double) * 2 * ne);
+    if (!rc) rc = fmbh_h2d(pl->d_edge, e, sizeof(double) * 2 * ne, stream);
+    free(gx);
+    free(gw);
+    free(e);
+    if (rc) goto fail;
+  }
+  /* Robin coefficients in Morton order */
+  if (robin) {
+    int N = t->nelem;
+    fmb_c64 *h = (fmb_c64 *)malloc(sizeof(fmb_c64) * N),
+            *hs = (fmb_c64 *)malloc(sizeof(fmb_c64) * N);
+    int *perm = (int *)malloc(sizeof(int) * N);
+    rc = fmbh_d2h(h, robin, sizeof(fmb_c64) * N, stream);
+    if (!rc) rc = fmbh_d2h(perm, t->d_perm, sizeof(int) * N, stream);
+    if (!rc)
+      for (int i = 0; i < N; i++) hs[i] = h[perm[i]];
+    if (!rc) rc = fmbh_malloc((void **)&d_robin, sizeof(fmb_c64) * N);
+    if (!rc) rc = fmbh_h2d(d_robin, hs, sizeof(fmb_c64) * N, stream);
+    free(h);
+    free(hs);
+    free(perm);
+    if (rc) goto fail;
+  }
+  if ((rc = fmbh_near_layout(pl, stream))) goto fail;
+  /* FP64 originals */
+  {
+    size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
+    double betas[FMB_NOFF];
+    for (int i = 0; i < pl->nbeta; i++) betas[i] = acos(pl->beta_cos[i]);
+    if ((rc = fmbh_malloc((void **)&d_H64, sizeof(double) * hs))) goto fail;
+    if ((rc = fmbh_gen_H(betas, pl->nbeta, pl->Pmax, d_H64, stream))) goto fail;
+    for (int l = 2; l <= L; l++) {
+      double side = t->D * ldexp(1.0, -l), tv[FMB_NDIST];
+      for (int d = 0; d < FMB_NDIST; d++) tv[d] = side * sqrt((double)k_dist2[d]);
+      size_t cs = (size_t)fmb_coax_size(pl->p[l]) * FMB_NDIST;
+      if ((rc = fmbh_malloc((void **)&d_sr64[l], sizeof(fmb_c64) * cs))) goto fail;
+      if ((rc = fmbh_gen_coax(FMB_COAX_SR, pl->k, tv, FMB_NDIST, pl->p[l],
+                              d_sr64[l], stream)))
+        goto fail;
+      if (l > 2) {
+        int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
+        double r = side * sqrt(3.0) / 2.0;
+        if ((rc = fmbh_malloc((void **)&d_rr64[l],
+                              sizeof(fmb_c64) * fmb_coax_size(P))))
+          goto fail;
+        if ((rc = fmbh_gen_coax(FMB_COAX_RR, pl->k, &r, 1, P, d_rr64[l], stream)))
+          goto fail;
+      }
+    }
+  }
+  pl->nops = prm->precision == FMB_MIXED ? 2 : 1;
+  for (int i = 0; i < pl->nops; i++) {
+    if ((rc = opset_alloc(pl, &pl->ops[i], i == 0 ? FMB_FP64 : FMB_MIXED))) goto fail;
+    if ((rc = build_opset(pl, &pl->ops[i], d_H64, d_sr64, d_rr64, d_robin)))
+      goto fail;
+  }
+  fmbh_free(d_H64);
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    fmbh_free(d_sr64[l]);
+    fmbh_free(d_rr64[l]);
+  }
+  fmbh_free(d_robin);
+  *out = pl;
+  return FMB_OK;
+fail:
+  fmbh_free(d_H64);
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    fmbh_free(d_sr64[l]);
+    fmbh_free(d_rr64[l]);
+  }
+  fmbh_free(d_robin);
+  fmb_plan_destroy(pl);
+  return rc;
+}
+
+int fmb_plan_orders(const fmb_plan *pl, int *p16) {
+  if (!pl || !p16) return fmb_set_error(FMB_ERR_ARG, "null plan");
+  for (int l = 0; l < 16; l++) p16[l] = l < FMB_MAXLEV ? pl->p[l] : 0;
+  return FMB_OK;
+}
+
+int fmb_plan_memory(const fmb_plan *pl, double *bytes) {
+  if (!pl || !bytes) return fmb_set_error(FMB_ERR_ARG, "null plan");
+  *bytes = (double)pl->bytes;
+  return FMB_OK;
+}
+
+int fmb_set_timing(fmb_plan *pl, int on) {
+  if (!pl) return fmb_set_error(FMB_ERR_ARG, "null plan");
+  pl->timing = on;
+  return FMB_OK;
+}
+
+int fmb_last_timing(const fmb_plan *pl, double *phase7, double *m2l16) {
+  if (!pl) return fmb_set_error(FMB_ERR_ARG, "null plan");
+  if (phase7) memcpy(phase7, pl->t_phase, sizeof(double) * 7);
+  if (m2l16)
+    for (int l = 0; l < 16; l++) m2l16[l] = l < FMB_MAXLEV ? pl->t_m2l[l] : 0.0;
+  return FMB_OK;
+}
+
+/* phase events: 0 start, 1 after gather+P2M, 2 after M2M, 3 after M2L/L2L,
+ * 4 after L2P+near */
+int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y, int nrhs,
+               void *stream) {
+  if (!pl || !x || !y || nrhs < 1 || nrhs > pl->nrhs_max)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_matvec: nrhs %d (max %d)", nrhs,
+                         pl ? pl->nrhs_max : 0);
+  int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
+  const fmb_tree *t = pl->tree;
+  fmb_opset *o = &pl->ops[op];
+  int rc, L = t->lmax;
+  void *ev[4 + 2 * FMB_MAXLEV] = {0};
+  int tm = pl->timing;
+  if (tm)
+    for (int i = 0; i < 4 + 2 * FMB_MAXLEV; i++)
+      if ((rc = fmbh_event_create(&ev[i]))) return rc;
+  if (tm) fmbh_event_record(ev[0], stream);
+  if ((rc = fmbh_gather(pl, op, x, nrhs, stream))) goto out;
+  for (int l = 2; l <= L; l++)
+    if ((rc = fmbh_memset(o->d_L[l], 0,
+                          (size_t)t->nbox[l] * nrhs * pl->p[l] * pl->p[l] * o->csz,
+                          stream)))
+      goto out;
+  if ((rc = fmbh_p2m(pl, op, nrhs, stream))) goto out;
+  if (tm) fmbh_event_record(ev[1], stream);
+  for (int l = L; l > 2; l--)
+    if ((rc = fmbh_m2m(pl, op, l, nrhs, stream))) goto out;
+  if (tm) fmbh_event_record(ev[2], stream);
+  for (int l = 2; l <= L; l++) {
+    if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
+    if ((rc = fmbh_m2l(pl, op, l, nrhs, stream))) goto out;
+    if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
+    if (l > 2 && (rc = fmbh_l2l(pl, op, l, nrhs, stream))) goto out;
+  }
+  if (tm) fmbh_event_record(ev[3], stream);
+  if ((rc = fmbh_l2p_near(pl, op, nrhs, stream))) goto out;
+  if ((rc = fmbh_scatter(pl, op, y, nrhs, stream))) goto out;
+  if (tm) {
+    void *e4;
+    fmbh_event_create(&e4);
+    fmbh_event_record(e4, stream);
+    float ms;
+    fmbh_event_elapsed(ev[0], ev[1], &ms);
+    pl->t_phase[1] = ms;
+    fmbh_event_elapsed(ev[1], ev[2], &ms);
+    pl->t_phase[2] = ms;
+    fmbh_event_elapsed(ev[2], ev[3], &ms);
+    pl->t_phase[3] = ms;
+    fmbh_event_elapsed(ev[3], e4, &ms);
+    pl->t_phase[5] = ms;
+    fmbh_event_elapsed(ev[0], e4, &ms);
+    pl->t_phase[6] = ms;
+    for (int l = 2; l <= L; l++) {
+      fmbh_event_elapsed(ev[4 + 2 * l], ev[5 + 2 * l], &ms);
+      pl->t_m2l[l] = ms;
+    }
+    fmbh_event_destroy(e4);
+  }
+out:
+  if (tm)
+    for (int i = 0; i < 4 + 2 * FMB_MAXLEV; i++) fmbh_event_destroy(ev[i]);
+  return rc;
+}
+
+int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,
+                       fmb_c64 *b, void *stream) {
+  if (!pl || !dirs || !b || nrhs < 1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_rhs_plane_wave: invalid arguments");
+  return fmbh_rhs_plane(pl, dirs, nrhs, b, stream);
+}

@@ -1,0 +1,197 @@
+/*
+ * fmbem.h -- C ABI of the MI355X (gfx950) fast multipole boundary element
+ * solver for 3-D Helmholtz acoustics.
+ *
+ * Every entry point has C linkage, takes plain pointers and sizes, and
+ * returns FMB_OK or a negative error code (text via fmb_last_error()).
+ * Streams are hipStream_t passed as void* (NULL = the default stream).
+ * Vectors handed to fmb_matvec / fmb_gmres / fmb_rhs_plane_wave live in
+ * device memory, as nrhs consecutive vectors of N complex FP64 values in the
+ * caller's element order.  Handles are thread-compatible: one handle per
+ * host thread at a time.
+ *
+ * Each declaration names the reference interface it replaces
+ * (ZiqiFrancisFan/FMBEM_GPU_C, file:line).
+ */
+#ifndef FMBEM_H
+#define FMBEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ basics */
+#define FMB_OK 0
+#define FMB_ERR_ARG (-1)     /* invalid argument                          */
+#define FMB_ERR_IO (-2)      /* file missing or malformed                 */
+#define FMB_ERR_HIP (-3)     /* HIP runtime error                         */
+#define FMB_ERR_NOMEM (-4)   /* allocation failed                         */
+#define FMB_ERR_DEPTH (-5)   /* s+1 centroids share a level-10 box        */
+#define FMB_ERR_NOCONV (-6)  /* GMRES did not reach the tolerance         */
+#define FMB_ERR_RANGE (-7)   /* order / size beyond a supported limit     */
+
+typedef struct { float re, im; } fmb_c32;
+typedef struct { double re, im; } fmb_c64;
+
+/* Replaces the HOST_CALL/CUDA_CALL "Error at file:line" prints
+ * (translation.h:64-126): message of the last failing call on this thread. */
+const char *fmb_last_error(void);
+int fmb_version(void);
+
+/* ------------------------------------------------------------ mesh */
+typedef struct fmb_mesh fmb_mesh;
+
+/* findNum (octree.c:563-584; octree.h:30): counts 'v' and 'f' lines. */
+int fmb_obj_count(const char *path, int *nv, int *nf);
+/* readOBJ (octree.c:586-619; octree.h:32): 'v x y z', 'f a b c' (1-based). */
+int fmb_mesh_load_obj(const char *path, fmb_mesh **out);
+/* Mesh from caller arrays: V[nv*3] doubles, F[nf*3] 0-based ids. */
+int fmb_mesh_from_arrays(const double *V, int nv, const int *F, int nf,
+                         fmb_mesh **out);
+/* Icosphere with a vertex at each pole; level 3, radius 0.1 gives the vertex
+ * set of the reference's sphere_100mm.obj.  Faces are wound so that the
+ * reference normal (n0-n2)x(n1-n2) (integral.cu:420) points outward. */
+int fmb_mesh_icosphere(int level, double radius, fmb_mesh **out);
+/* Ellipsoid (semi-axes a,b,c) from a geodesic sphere of frequency nu
+ * (20 nu^2 faces), radially perturbed by a real spherical-harmonic field of
+ * degrees lmin..lmax with N(0,1) coefficients (seeded) scaled to rms_frac
+ * of the radius. */
+int fmb_mesh_ellipsoid(int nu, double a, double b, double c, int lmin,
+                       int lmax, double rms_frac, unsigned long long seed,
+                       fmb_mesh **out);
+/* Re-winds faces so the reference normal points away from the enclosed
+ * volume (the shipped sphere meshes point inward). Returns faces flipped. */
+int fmb_mesh_orient_outward(fmb_mesh *m, int *nflipped);
+int fmb_mesh_sizes(const fmb_mesh *m, int *nv, int *nf);
+int fmb_mesh_export(const fmb_mesh *m, double *V, int *F);
+void fmb_mesh_destroy(fmb_mesh *m);
+
+/* ------------------------------------------------------------ octree */
+typedef struct fmb_tree fmb_tree;
+
+/* srcBoxes + FMMLevelSet + FMMLevelSetNumSR + the index arrays of genOctree
+ * (octree.c:749-768, 890-919; translation.cu:3007-3076, 2797-2969), built on
+ * the device as Morton-sorted arrays.  s: max elements per leaf box
+ * (deterLmax, octree.c:635-679) used when depth == 0; depth > 0 fixes the
+ * leaf level.  lmin is 2 (translation.cu:2976). */
+int fmb_tree_build(const fmb_mesh *m, int s, int depth, void *stream,
+                   fmb_tree **out);
+/* lmax, D (cube side), pmin (cube corner, 3 doubles) (struct octree,
+ * structs.h:93-150). */
+int fmb_tree_info(const fmb_tree *t, int *lmin, int *lmax, double *D,
+                  double *pmin);
+/* FMMLevelSet (octree.c:890-906): sorted Morton keys of the non-empty boxes
+ * at level l; keys may be NULL to query the count. */
+int fmb_tree_level(const fmb_tree *t, int l, int *nbox, int *keys);
+/* FMMLevelSetNumSR (octree.c:908-919): |I4(b) & level set| per box. */
+int fmb_tree_num_sr(const fmb_tree *t, int l, int *num);
+/* btmLvlElemIdx (translation.cu:3062-3076): leaf-box index per element, and
+ * the Morton order of the elements (perm[i] = original id of sorted i). */
+int fmb_tree_elem_box(const fmb_tree *t, int *box, int *perm);
+/* M2L pairs of level l (srTransOriginArr / srTransDestArr,
+ * translation.cu:2847-2927) as box indices, offset-major order; offsets are
+ * codes (dx+3)*49+(dy+3)*7+(dz+3) of target-minus-source grid steps. */
+int fmb_tree_pairs(const fmb_tree *t, int l, int *npairs, int *tgt, int *src,
+                   int *off);
+void fmb_tree_destroy(fmb_tree *t);
+
+/* ------------------------------------------------------------ operators */
+/* truncNum (octree.c:777-796; octree.h:112). */
+int fmb_trunc_num(double k, double eps, double sigma, double a);
+int fmb_rot_size(int p);  /* sparseRotSize, translation.cu:12-15       */
+int fmb_coax_size(int p); /* sparseCoaxTransSize, translation.cu:17-20 */
+/* genSparseRotMats (translation.cu:1297-1394; translation.h:224): angles is
+ * n triples (alpha, beta, gamma); out (host) receives n*fmb_rot_size(p)
+ * complex values in the reference n-block layout.  Generated on the device
+ * in FP64 with O(p^3) work and memory. */
+int fmb_gen_rot_mats(const double *angles, int n, int p, fmb_c64 *out);
+/* genSRSparseCoaxTransMat / genRRSparseCoaxTransMat (translation.cu:982-1062,
+ * 822-903; translation.h:174-187): kind 0 = (R|R), 1 = (S|R); n distances t,
+ * out (host) receives n*fmb_coax_size(p) values in the |m|-block layout. */
+#define FMB_COAX_RR 0
+#define FMB_COAX_SR 1
+int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
+                      fmb_c64 *out);
+
+/* ------------------------------------------------------------ plan */
+#define FMB_FP64 0  /* plain path: FP64 everywhere                       */
+#define FMB_MIXED 1 /* FP32 translations / near field / bases, FP64
+                       recurrences, singular terms, Arnoldi, refinement  */
+
+#define FMB_PRULE_TRUNCNUM 0 /* truncNum(k, eps, sigma, side) per level    */
+#define FMB_PRULE_EXCESS 1   /* ka_box + 1.8 log10(ka_box + pi), ka_box =
+                                k * box diameter (sqrt(3) * side)          */
+#define FMB_PRULE_EXPLICIT 2 /* p_levels[l] as given                       */
+
+#define FMB_RULE_COLLAPSED 0 /* INTORDER^2 collapsed rule, integral.cu:80-139 */
+#define FMB_RULE_GAUSS7 1    /* 7-point degree-5 triangle rule              */
+
+typedef struct fmb_plan fmb_plan;
+
+typedef struct fmb_plan_params {
+  double k;                  /* wavenumber (rad/m)                          */
+  double alpha_re, alpha_im; /* Burton-Miller coupling; 0,0 selects i/k     */
+  int precision;             /* FMB_FP64 or FMB_MIXED                       */
+  int p_rule;                /* FMB_PRULE_*                                 */
+  double eps, sigma;         /* truncNum parameters (0.05, 1.5)             */
+  int p_levels[16];          /* FMB_PRULE_EXPLICIT: order at level l        */
+  int p_min;                 /* lower clamp on every level's order          */
+  int quad_rule, quad_order; /* panel rule for far pairs and P2M            */
+  int near_rule, near_order; /* refined rule for close pairs                */
+  double near_ratio;         /* close: |x_i - c_j| < ratio * circumradius_j */
+  int nedge;                 /* Gauss points per edge, singular self terms  */
+  int nrhs_max;              /* right-hand sides per matvec                 */
+} fmb_plan_params;
+
+void fmb_plan_params_default(fmb_plan_params *prm);
+
+/* genOctree's operator part + initOctree defaults (translation.cu:2971-3348):
+ * per-level orders, rotation matrices per unique polar angle, (S|R) and
+ * (R|R) coaxial matrices, P2M/L2P bases, near-field blocks (regular rule,
+ * refined close rule, analytic singular self terms).  robin (device or NULL)
+ * gives sigma per element (caller order) for dp/dn = sigma p; NULL: rigid. */
+int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
+                    const fmb_c64 *robin, void *stream, fmb_plan **out);
+int fmb_plan_orders(const fmb_plan *pl, int *p16);
+/* Device bytes held by the plan. */
+int fmb_plan_memory(const fmb_plan *pl, double *bytes);
+
+/* OprL (translation.cu:3412-3769), completed: y = A x for the Burton-Miller
+ * operator (near field + P2M, M2M, M2L, L2L, L2P) on nrhs vectors.  On an
+ * FMB_MIXED plan prec selects FMB_MIXED (FP32 pass) or FMB_FP64. */
+int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y,
+               int nrhs, void *stream);
+
+/* Right-hand side p_inc + alpha dp_inc/dn for unit plane waves e^{ik d.x}
+ * (dirs: nrhs*3 host doubles).  The reference has none. */
+int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,
+                       fmb_c64 *b, void *stream);
+
+typedef struct fmb_gmres_info {
+  int iters;            /* inner Arnoldi steps (max over rhs)           */
+  int matvecs_fp64;     /* FP64 FMM passes                              */
+  int matvecs_work;     /* working-precision FMM passes                 */
+  int restarts;
+  double relres[64];    /* final true relative residual per rhs (<=64)  */
+  double seconds;
+} fmb_gmres_info;
+
+/* GMRES(restart) to ||b - A x|| <= tol ||b|| (FP64 true residual), FP64
+ * Arnoldi with classical Gram-Schmidt and reorthogonalisation.  On a
+ * FMB_MIXED plan: iterative refinement with FP64 residuals around GMRES
+ * on the FP32 pass.  x holds the initial guess.  The reference has none. */
+int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
+              int restart, double tol, int maxit, fmb_gmres_info *info,
+              void *stream);
+void fmb_plan_destroy(fmb_plan *pl);
+
+/* Per-phase timing of the last fmb_matvec (ms): near, p2m, m2m, m2l, l2l,
+ * l2p, total; and the M2L kernel's event time per level. */
+int fmb_last_timing(const fmb_plan *pl, double *phase7, double *m2l16);
+/* Enables HIP-event timing of each phase inside fmb_matvec. */
+int fmb_set_timing(fmb_plan *pl, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

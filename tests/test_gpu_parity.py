@@ -1,0 +1,119 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the
+same inputs.  Tolerances: tree/index work bit-exact; FP64 operators 1e-11
+relative; FP64 FMM matvec 1e-9 relative to the oracle FMM (same algorithm,
+different summation order); mixed path 1e-5 relative to FP64."""
+import numpy as np
+import pytest
+import torch
+
+import fmbem_gpu_c_amd as F
+import oracle as O
+from conftest import wavenumber
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_tree(V, Fc, s=5, depth=0):
+    return O.Tree(V, Fc, s=s, depth=depth)
+
+
+def test_tree_matches_oracle(sphere_l3):
+    m, V, Fc = sphere_l3
+    t = F.Tree(m, s=5)
+    ot = _oracle_tree(V, Fc, s=5)
+    lmin, lmax, D, pmin = t.info()
+    assert lmax == ot.lmax
+    oD, opmin = ot.geom()
+    assert D == pytest.approx(oD, rel=1e-15)
+    np.testing.assert_allclose(pmin, opmin, rtol=0, atol=1e-15)
+    for l in range(2, lmax + 1):
+        np.testing.assert_array_equal(t.level(l), ot.keys(l))
+        np.testing.assert_array_equal(t.num_sr(l), ot.num_sr(l))
+        keys = t.level(l)
+        tg, sr, off = t.pairs(l)
+        got = set(zip(keys[tg].tolist(), keys[sr].tolist()))
+        want = set()
+        kset = set(keys.tolist())
+        for b in keys:
+            for s in O.I4(int(b), l):
+                if int(s) in kset:
+                    want.add((int(b), int(s)))
+        assert got == want
+    box, perm = t.elem_box()
+    np.testing.assert_array_equal(perm, ot.perm())
+
+
+@pytest.mark.parametrize("p", [8, 17, 40])
+def test_rot_mats_match_oracle(p):
+    rng = np.random.default_rng(p)
+    ang = np.column_stack([rng.uniform(-np.pi, np.pi, 5), rng.uniform(0, np.pi, 5),
+                           rng.uniform(-np.pi, np.pi, 5)])
+    got = F.gen_rot_mats(ang, p)
+    for i in range(5):
+        want = O.rot_sparse(p, *ang[i])
+        err = np.max(np.abs(got[i] - want)) / np.max(np.abs(want))
+        assert err < 1e-11, (i, err)
+
+
+@pytest.mark.parametrize("kind,p,k,t", [(O.COAX_RR, 12, 18.3, 0.02), (O.COAX_SR, 16, 18.3, 0.1),
+                                        (O.COAX_SR, 30, 91.6, 0.3), (O.COAX_RR, 30, 91.6, 0.05)])
+def test_coax_mats_match_oracle(kind, p, k, t):
+    got = F.gen_coax_mats(kind, k, [t, 1.5 * t], p)
+    for i, tt in enumerate([t, 1.5 * t]):
+        want = O.coax_sparse(kind, p, k, tt)
+        err = np.max(np.abs(got[i] - want) / np.maximum(np.abs(want), 1e-300 + np.abs(want).max() * 1e-14))
+        assert err < 1e-9, (i, err)
+
+
+def _config1_pair(sphere_l3, precision=F.FMB_FP64, nrhs=1):
+    m, V, Fc = sphere_l3
+    k = wavenumber(1000.0)
+    t = F.Tree(m, s=5)
+    pl = F.Plan(t, k, precision=precision, nrhs_max=nrhs)
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(s=5)
+    return t, pl, ob, k
+
+
+def test_matvec_fp64_matches_oracle(sphere_l3):
+    t, pl, ob, k = _config1_pair(sphere_l3)
+    L, op = ob.orders()
+    np.testing.assert_array_equal(pl.orders()[2:L + 1], op[2:L + 1])
+    rng = np.random.default_rng(7)
+    n = ob.n
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = pl.matvec(torch.tensor(x, device="cuda")).cpu().numpy()
+    yo = ob.fmm_matvec(x)[0]
+    err = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    assert err < 1e-9, err
+    yd = ob.dense_matvec(x)[0]
+    assert np.linalg.norm(y - yd) / np.linalg.norm(yd) < 1e-4
+
+
+def test_matvec_mixed_vs_fp64(sphere_l3):
+    t, pl, ob, k = _config1_pair(sphere_l3, precision=F.FMB_MIXED, nrhs=2)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((2, ob.n)) + 1j * rng.standard_normal((2, ob.n))
+    xt = torch.tensor(x, device="cuda")
+    y64 = pl.matvec(xt, prec=F.FMB_FP64).cpu().numpy()
+    y32 = pl.matvec(xt, prec=F.FMB_MIXED).cpu().numpy()
+    yo = ob.fmm_matvec(x)
+    assert np.linalg.norm(y64 - yo) / np.linalg.norm(yo) < 1e-9
+    assert np.linalg.norm(y32 - y64) / np.linalg.norm(y64) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [F.FMB_FP64, F.FMB_MIXED])
+def test_gmres_config1(sphere_l3, precision):
+    t, pl, ob, k = _config1_pair(sphere_l3, precision=precision)
+    b = pl.rhs_plane_wave([0.0, 0.0, 1.0])
+    np.testing.assert_allclose(b[0].cpu().numpy(), ob.rhs_plane([0, 0, 1.0]), rtol=1e-13, atol=1e-13)
+    x, info = pl.gmres(b, restart=30, tol=1e-6)
+    assert info.relres[0] <= 1e-6
+    u = x[0].cpu().numpy()
+    xo, it, rr, rc = ob.gmres(ob.rhs_plane([0, 0, 1.0]), use_fmm=True)
+    assert rc == 0
+    assert np.linalg.norm(u - xo) / np.linalg.norm(xo) < 1e-5
+    c, nrm, area = ob.centroids()
+    ct = c[:, 2] / np.linalg.norm(c, axis=1)
+    mie = O.mie_rigid(k * 0.1, ct)
+    assert np.linalg.norm(u - mie) / np.linalg.norm(mie) < 0.02
