@@ -97,6 +97,7 @@ _SIGS = {
     "fmb_plan_orders": (ctypes.c_int, [_vp, _ip]),
     "fmb_plan_memory": (ctypes.c_int, [_vp, _dp]),
     "fmb_matvec": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "fmb_matvec_parts": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.POINTER(GmresInfo), _vp]),
@@ -287,6 +288,13 @@ class Plan:
         if y is None:
             y = torch.empty_like(x)
         _check(lib().fmb_matvec(self.h, prec, _dev(x), _dev(y), nrhs, _stream()))
+        return y
+
+    def matvec_parts(self, x, parts, prec=FMB_FP64):
+        import torch
+        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        y = torch.empty_like(x)
+        _check(lib().fmb_matvec_parts(self.h, prec, parts, _dev(x), _dev(y), nrhs, _stream()))
         return y
 
     def rhs_plane_wave(self, dirs):

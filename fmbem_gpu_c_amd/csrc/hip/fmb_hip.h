@@ -72,7 +72,8 @@ int fmbh_p2m(const fmb_plan *pl, int op, int nrhs, void *stream);
 int fmbh_m2m(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_l2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
-int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, void *stream);
+int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
+                  void *stream);
 int fmbh_mmajor_table(int p, int **d_tab, void *stream);
 
 /* ---- FP64 vector kernels for GMRES (blas.hip) */

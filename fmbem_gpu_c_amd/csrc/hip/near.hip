@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256)
                const unsigned long long *__restrict__ boff,
                const cplx<T> *__restrict__ A, const cplx<T> *__restrict__ E,
                const cplx<T> *__restrict__ Lc, const cplx<T> *__restrict__ xs,
-               cplx<T> *__restrict__ ys, int N, int ps, int nrhs) {
+               cplx<T> *__restrict__ ys, int N, int ps, int nrhs, int parts) {
   extern __shared__ int colj[];
   int b = blockIdx.x;
   int r0 = bstart[b], nr = bstart[b + 1] - r0, nc = ncols[b];
@@ -222,14 +222,14 @@ __global__ void __launch_bounds__(256)
     T sr = 0, si = 0;
     const cplx<T> *arow = Ab + (size_t)r * nc;
     const cplx<T> *x = xs + (size_t)rh * N;
-    for (int c = lane; c < nc; c += 64) {
+    for (int c = (parts & 1) ? lane : nc; c < nc; c += 64) {
       cplx<T> a = arow[c], v = x[colj[c]];
       sr += a.re * v.re - a.im * v.im;
       si += a.re * v.im + a.im * v.re;
     }
     const cplx<T> *e = E + (size_t)i * ps;
     const cplx<T> *l = Lc + ((size_t)b * nrhs + rh) * ps;
-    for (int f = lane; f < ps; f += 64) {
+    for (int f = (parts & 2) ? lane : ps; f < ps; f += 64) {
       cplx<T> a = e[f], v = l[f];
       sr += a.re * v.re - a.im * v.im;
       si += a.re * v.im + a.im * v.re;
@@ -314,7 +314,7 @@ extern "C" int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *robin,
   return FMB_OK;
 }
 
-extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs,
+extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
                              void *stream) {
   const fmb_tree *t = pl->tree;
   hipStream_t st = (hipStream_t)stream;
@@ -329,7 +329,7 @@ extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs,
                        (const cplx<double> *)o->d_L2P,
                        (const cplx<double> *)o->d_L[L],
                        (const cplx<double> *)o->d_xs, (cplx<double> *)o->d_ys,
-                       t->nelem, pL * pL, nrhs);
+                       t->nelem, pL * pL, nrhs, parts);
   else
     hipLaunchKernelGGL(k_l2p_near<float>, dim3(nb), dim3(256), lds, st,
                        t->d_box_start, t->d_near_start, t->d_near_box,
@@ -337,7 +337,7 @@ extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs,
                        (const cplx<float> *)o->d_L2P,
                        (const cplx<float> *)o->d_L[L],
                        (const cplx<float> *)o->d_xs, (cplx<float> *)o->d_ys,
-                       t->nelem, pL * pL, nrhs);
+                       t->nelem, pL * pL, nrhs, parts);
   FMB_LAUNCH_CHECK();
   return FMB_OK;
 }

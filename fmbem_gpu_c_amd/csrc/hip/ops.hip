@@ -2,119 +2,89 @@
 // device with O(p^3) work and O(p^2) scratch per matrix, and their scaled
 // conversion to the compute precision.
 //
-//   rotation H_n^{m',m}(beta): initial column from the normalised Legendre
-//     functions (rotMatsInit, translation.cu:1064-1092), recurrence in m
-//     (sparseRotMatGen, translation.cu:1179-1204), symmetry for m < 0
-//     (translation.cu:1206-1219); phases e^{i m alpha} e^{-i m' gamma}
-//     (translation.cu:1229) are applied on the fly by the translation kernels.
+//   rotation H_n^{m',m}(beta) in the normalisation of rotMatsInit /
+//     sparseRotMatGen (translation.cu:1064-1237), computed from Wigner d by the
+//     recurrence in the degree (stable; the reference's recurrence in m is
+//     not); phases e^{i m alpha} e^{-i m' gamma} (translation.cu:1229) are
+//     applied on the fly by the translation kernels.
 //   coaxial (R|R), (S|R): initial column (-1)^n' sqrt(2n'+1) f_n'(kt)
 //     (translation.cu:555-584, 919-949), sectorial recurrence
 //     (translation.cu:689-711), advancing in n (translation.cu:713-745),
 //     |m|-block storage (translation.cu:760-769).
-// One workgroup per matrix (rotation, sectorial pass) or per |m| block
-// (advancing pass); the recurrences run sequentially in m (resp. n) with the
-// orthogonal index spread over the workgroup.
+// Rotation: one lane per (m', m) column.  Coaxial: one workgroup per matrix
+// (sectorial pass) or per |m| block (advancing pass), sequential in m
+// (resp. n) with the orthogonal index spread over the workgroup.
 #include "fmb_device.h"
 
 namespace {
 
 constexpr int NT = 256;
 
+// Wigner d^l_{m'm}(beta) from its closed form at l0 = max(|m'|,|m|)
+__device__ double d_top(int j, int m, double beta) {
+  double s, c;
+  sincos(0.5 * beta, &s, &c);
+  double lv = 0.5 * (lgamma(2.0 * j + 1) - lgamma(j + m + 1.0) - lgamma(j - m + 1.0));
+  double sg = ((j - m) & 1) ? -1.0 : 1.0;
+  if (j + m > 0) {
+    if (c == 0.0) return 0.0;
+    lv += (j + m) * log(fabs(c));
+    if (c < 0 && ((j + m) & 1)) sg = -sg;
+  }
+  if (j - m > 0) {
+    if (s == 0.0) return 0.0;
+    lv += (j - m) * log(fabs(s));
+    if (s < 0 && ((j - m) & 1)) sg = -sg;
+  }
+  return sg * exp(lv);
+}
+
+__device__ double d_start(int mp, int m, double beta) {
+  int j = abs(mp) > abs(m) ? abs(mp) : abs(m);
+  if (mp == j) return d_top(j, m, beta);
+  if (mp == -j) return ((j + m) & 1 ? -1.0 : 1.0) * d_top(j, -m, beta);
+  if (m == j) return ((j - mp) & 1 ? -1.0 : 1.0) * d_top(j, mp, beta);
+  return d_top(j, -mp, beta);
+}
+
+// H_n^{m',m}(beta) = eps_{m'} (-1)^m eps_m d^n_{m'm}(beta), one lane per
+// (m', m) running the three-term recurrence in the degree n.  The reference's
+// recurrence in m (translation.cu:1179-1204) loses about a digit per degree
+// past n ~ 15 and is not used.
 __global__ void __launch_bounds__(NT)
-    k_gen_H(const double *betas, int P, double *H, double *scratch) {
-  int b = blockIdx.x;
+    k_gen_H(const double *betas, int P, double *H) {
+  int b = blockIdx.y;
+  int W = 2 * P - 1;
+  int idx = blockIdx.x * NT + threadIdx.x;
+  if (idx >= W * W) return;
+  int mp = idx % W - (P - 1), m = idx / W - (P - 1);
   double beta = betas[b];
-  int nmax = 2 * P - 2;
-  int sz = (nmax + 1) * (nmax + 1);
-  double *col = scratch + (size_t)b * 2 * sz;
-  double *nxt = col + sz;
+  double cb = cos(beta);
   double *Hb = H + (size_t)b * rot_off(P);
-  double cb, sb;
-  sincos(beta, &sb, &cb);
-  sb = fabs(sb);
-  __shared__ double diag[512];
-  if (threadIdx.x == 0) {
-    double v = 1.0 / sqrt(4.0 * M_PI);
-    diag[0] = v;
-    for (int m = 1; m <= nmax; m++) {
-      v *= sqrt((2.0 * m + 1.0) / (2.0 * m)) * sb;
-      diag[m] = v;
+  int l0 = abs(mp) > abs(m) ? abs(mp) : abs(m);
+  double em = (m < 0 && ((-m) & 1)) ? -1.0 : 1.0;
+  double ep = (mp > 0 && (mp & 1)) ? -1.0 : 1.0;
+  double f = em * ep;
+  double dm2 = 0.0, dm1 = d_start(mp, m, beta);
+  for (int n = l0; n < P; n++) {
+    double v;
+    if (n == l0)
+      v = dm1;
+    else if (n == 1)
+      v = cb;
+    else {
+      double a = (2.0 * n - 1.0) * (n * (n - 1.0) * cb - (double)mp * m);
+      double bq = n * sqrt(fmax(0.0, ((n - 1.0) * (n - 1.0) - (double)mp * mp) *
+                                         ((n - 1.0) * (n - 1.0) - (double)m * m)));
+      double den = (n - 1.0) * sqrt(((double)n * n - (double)mp * mp) *
+                                    ((double)n * n - (double)m * m));
+      v = (a * dm1 - bq * dm2) / den;
     }
-  }
-  __syncthreads();
-  // initial column H_n^{m',0} = sqrt(4pi/(2n+1)) P~_n^{|m'|}(cos beta)
-  for (int m = threadIdx.x; m <= nmax; m += NT) {
-    double p2 = 0.0, p1 = diag[m];
-    for (int n = m; n <= nmax; n++) {
-      double pv;
-      if (n == m)
-        pv = p1;
-      else if (n == m + 1)
-        pv = sqrt(2.0 * m + 3.0) * cb * p1;
-      else {
-        double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
-        double bq = sqrt((((double)(n - 1) * (n - 1)) - (double)m * m) /
-                         (4.0 * (n - 1) * (n - 1) - 1.0));
-        pv = a * (cb * p1 - bq * p2);
-      }
-      if (n > m) {
-        p2 = p1;
-        p1 = pv;
-      }
-      double hv = sqrt(4.0 * M_PI / (2.0 * n + 1.0)) * pv;
-      col[n * n + n + m] = hv;
-      col[n * n + n - m] = hv;
+    if (n > l0) {
+      dm2 = dm1;
+      dm1 = v;
     }
-  }
-  __syncthreads();
-  for (int f = threadIdx.x; f < P * P; f += NT) {
-    int n = (int)sqrt((double)f);
-    while (n * n > f) n--;
-    while ((n + 1) * (n + 1) <= f) n++;
-    int mp = f - n * n - n;
-    Hb[rot_off(n) + n * (2 * n + 1) + (mp + n)] = col[f];
-  }
-  for (int m = 0; m + 1 <= P - 1; m++) {
-    int nlo = m + 1, nhi = nmax - (m + 1);
-    int g0 = nlo * nlo, g1 = (nhi + 1) * (nhi + 1);
-    double bd_unused = 0.0;
-    (void)bd_unused;
-    for (int g = g0 + threadIdx.x; g < g1; g += NT) {
-      int n = (int)sqrt((double)g);
-      while (n * n > g) n--;
-      while ((n + 1) * (n + 1) <= g) n++;
-      int mp = g - n * n - n;
-      double t0 = 0.5 * bcoef(n + 1, -mp - 1) * (1.0 - cb);
-      double t1 = 0.5 * bcoef(n + 1, mp - 1) * (1.0 + cb);
-      double t2 = acoef(n, mp) * sb;
-      int b1 = (n + 1) * (n + 1) + (n + 1);
-      double v = t0 * col[b1 + mp + 1] - t1 * col[b1 + mp - 1] -
-                 t2 * col[b1 + mp];
-      nxt[g] = v / bcoef(n + 1, m);
-    }
-    __syncthreads();
-    double *tmp = col;
-    col = nxt;
-    nxt = tmp;
-    int q0 = (m + 1) * (m + 1);
-    for (int g = q0 + threadIdx.x; g < P * P; g += NT) {
-      int n = (int)sqrt((double)g);
-      while (n * n > g) n--;
-      while ((n + 1) * (n + 1) <= g) n++;
-      int mp = g - n * n - n;
-      Hb[rot_off(n) + (m + 1 + n) * (2 * n + 1) + (mp + n)] = col[g];
-    }
-    __syncthreads();
-  }
-  // m < 0: H_n^{m',m} = H_n^{-m',-m}
-  for (int g = threadIdx.x; g < P * P; g += NT) {
-    int n = (int)sqrt((double)g);
-    while (n * n > g) n--;
-    while ((n + 1) * (n + 1) <= g) n++;
-    int mp = g - n * n - n;
-    int ld = 2 * n + 1, off = rot_off(n);
-    for (int m = -n; m < 0; m++)
-      Hb[off + (m + n) * ld + (mp + n)] = Hb[off + (-m + n) * ld + (-mp + n)];
+    Hb[rot_off(n) + (m + n) * (2 * n + 1) + (mp + n)] = f * v;
   }
 }
 
@@ -186,6 +156,17 @@ __global__ void __launch_bounds__(NT)
       blk[(size_t)(n + 1 - m) * bl + (np - m)] = cur[np];
     __syncthreads();
   }
+  // entries with n' < n from the exact symmetry (E|F)^m_{n'n} =
+  // (-1)^{n+n'} (E|F)^m_{nn'}: the advancing step loses accuracy there when
+  // kt is small (it subtracts terms (kt)^-2 larger than its result)
+  for (int e = threadIdx.x; e < bl * bl; e += NT) {
+    int r = e % bl, c = e / bl;
+    if (r < c) {
+      double2 v = blk[(size_t)r * bl + c];
+      double sg = ((r + c) & 1) ? -1.0 : 1.0;
+      blk[e] = make_double2(sg * v.x, sg * v.y);
+    }
+  }
 }
 
 template <typename T>
@@ -216,10 +197,9 @@ __global__ void k_rot_phase(const double *H, int P, const double *ang,
   double al = ang[3 * a], ga = ang[3 * a + 2];
   const double *Hb = H + (size_t)a * rot_off(P);
   double2 *o = out + (size_t)a * rot_off(P);
-  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < P * P * P;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < rot_off(P);
        f += gridDim.x * blockDim.x) {
-    // enumerate (n, m', m) over the sparse layout by flat sparse index
-    if (f >= rot_off(P)) break;
+    // flat sparse index -> (n, m', m)
     int n = 0;
     while (rot_off(n + 1) <= f) n++;
     int e = f - rot_off(n), ld = 2 * n + 1;
@@ -239,17 +219,16 @@ extern "C" int fmbh_gen_H(const double *h_beta, int nbeta, int P, double *d_H,
   hipStream_t st = (hipStream_t)stream;
   if (P < 1 || 2 * P - 2 >= 512)
     return fmb_set_error(FMB_ERR_RANGE, "rotation order %d out of range", P);
-  double *d_b, *scr;
-  size_t sz = (size_t)(2 * P - 1) * (2 * P - 1);
+  double *d_b;
+  int W = 2 * P - 1;
   int rc = fmbh_malloc((void **)&d_b, sizeof(double) * nbeta);
-  if (!rc) rc = fmbh_malloc((void **)&scr, sizeof(double) * 2 * sz * nbeta);
   if (rc) return rc;
   if ((rc = fmbh_h2d(d_b, h_beta, sizeof(double) * nbeta, stream))) return rc;
-  hipLaunchKernelGGL(k_gen_H, dim3(nbeta), dim3(NT), 0, st, d_b, P, d_H, scr);
+  hipLaunchKernelGGL(k_gen_H, dim3((W * W + NT - 1) / NT, nbeta), dim3(NT), 0, st,
+                     d_b, P, d_H);
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize(st));
   fmbh_free(d_b);
-  fmbh_free(scr);
   return FMB_OK;
 }
 

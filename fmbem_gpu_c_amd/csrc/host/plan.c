@@ -428,7 +428,12 @@ int fmb_last_timing(const fmb_plan *pl, double *phase7, double *m2l16) {
  * 4 after L2P+near */
 int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y, int nrhs,
                void *stream) {
-  if (!pl || !x || !y || nrhs < 1 || nrhs > pl->nrhs_max)
+  return fmb_matvec_parts(pl, prec, 3, x, y, nrhs, stream);
+}
+
+int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
+                     fmb_c64 *y, int nrhs, void *stream) {
+  if (!pl || parts < 1 || parts > 3 || !x || !y || nrhs < 1 || nrhs > pl->nrhs_max)
     return fmb_set_error(FMB_ERR_ARG, "fmb_matvec: nrhs %d (max %d)", nrhs,
                          pl ? pl->nrhs_max : 0);
   int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
@@ -459,7 +464,7 @@ int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y, int nrhs,
     if (l > 2 && (rc = fmbh_l2l(pl, op, l, nrhs, stream))) goto out;
   }
   if (tm) fmbh_event_record(ev[3], stream);
-  if ((rc = fmbh_l2p_near(pl, op, nrhs, stream))) goto out;
+  if ((rc = fmbh_l2p_near(pl, op, nrhs, parts, stream))) goto out;
   if ((rc = fmbh_scatter(pl, op, y, nrhs, stream))) goto out;
   if (tm) {
     void *e4;

@@ -162,6 +162,11 @@ int fmb_plan_memory(const fmb_plan *pl, double *bytes);
 int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y,
                int nrhs, void *stream);
 
+/* The two halves of fmb_matvec: parts = 1 near field only, 2 far field only
+ * (P2M .. L2P: the part the reference's OprL computes), 3 both. */
+int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
+                     fmb_c64 *y, int nrhs, void *stream);
+
 /* Right-hand side p_inc + alpha dp_inc/dn for unit plane waves e^{ik d.x}
  * (dirs: nrhs*3 host doubles).  The reference has none. */
 int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,

@@ -57,6 +57,7 @@ _SIGS = {
     "fo_coax_size": (_ci, [_ci]),
     "fo_coax_block": (_ci, [_ci, _ci]),
     "fo_rot_H": (None, [_ci, _cd, _dp]),
+    "fo_rot_H_ref": (None, [_ci, _cd, _dp]),
     "fo_rot_sparse": (None, [_ci, _cd, _cd, _cd, _dp, _dp]),
     "fo_coax_sparse": (None, [_ci, _ci, _cd, _cd, _dp, _dp]),
     "fo_rot_apply": (None, [_ci, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -259,6 +260,12 @@ class Tree:
 def rot_H(p, beta):
     H = np.zeros(lib().fo_rot_size(p))
     lib().fo_rot_H(p, beta, H)
+    return H
+
+
+def rot_H_ref(p, beta):
+    H = np.zeros(lib().fo_rot_size(p))
+    lib().fo_rot_H_ref(p, beta, H)
     return H
 
 

@@ -94,6 +94,7 @@ int fo_rot_size(int p);
 int fo_coax_size(int p);
 int fo_coax_block(int p, int m);
 void fo_rot_H(int p, double beta, double *H);
+void fo_rot_H_ref(int p, double beta, double *H);
 void fo_rot_sparse(int p, double alpha, double beta, double gamma,
                    double *re, double *im);
 #define FO_COAX_RR 0

@@ -40,9 +40,77 @@ int fo_coax_block(int p, int m) {
   return a * (2 * a * a - (3 + 6 * p) * a + 1 + 6 * (p + p * p)) / 6;
 }
 
+/* Wigner d^l_{m'm}(beta) for l = l0..L, l0 = max(|m'|,|m|), by the
+ * three-term recurrence in the degree (Edmonds 1957, eq. 4.1.23 family),
+ * started from the closed form at l0.  Stable for all orders used here,
+ * unlike the m-recurrence of the reference (see fo_rot_H_ref). */
+static double d_top(int j, int m, double beta) {
+  double c = cos(0.5 * beta), s = sin(0.5 * beta);
+  double lv = 0.5 * (lgamma(2.0 * j + 1) - lgamma(j + m + 1.0) - lgamma(j - m + 1.0));
+  double sg = ((j - m) & 1) ? -1.0 : 1.0;
+  if (j + m > 0) {
+    if (c == 0.0) return 0.0;
+    lv += (j + m) * log(fabs(c));
+    if (c < 0 && ((j + m) & 1)) sg = -sg;
+  }
+  if (j - m > 0) {
+    if (s == 0.0) return 0.0;
+    lv += (j - m) * log(fabs(s));
+    if (s < 0 && ((j - m) & 1)) sg = -sg;
+  }
+  return sg * exp(lv);
+}
+
+static double d_start(int mp, int m, double beta) {
+  int j = abs(mp) > abs(m) ? abs(mp) : abs(m);
+  if (mp == j) return d_top(j, m, beta);
+  if (mp == -j) return ((j + m) & 1 ? -1.0 : 1.0) * d_top(j, -m, beta);
+  if (m == j) return ((j - mp) & 1 ? -1.0 : 1.0) * d_top(j, mp, beta);
+  return d_top(j, -mp, beta); /* m == -j */
+}
+
 /* H_n^{m',m}(beta) for n < p in the sparse layout: block n at n(4n^2-1)/3,
- * element (m'+n, m+n) column-major with leading dimension 2n+1. */
+ * element (m'+n, m+n) column-major with leading dimension 2n+1.
+ * H_n^{m',m} = eps_{m'} (-1)^m eps_m d^n_{m'm}(beta), eps_m = (-1)^m for
+ * m >= 0 and 1 for m < 0: the normalisation of rotMatsInit/sparseRotMatGen
+ * (translation.cu:1064-1237), checked against fo_rot_H_ref at small p. */
 void fo_rot_H(int p, double beta, double *H) {
+  double cb = cos(beta);
+  for (int mp = -(p - 1); mp <= p - 1; mp++)
+    for (int m = -(p - 1); m <= p - 1; m++) {
+      int l0 = abs(mp) > abs(m) ? abs(mp) : abs(m);
+      double em = (m < 0 ? ((-m) & 1 ? -1.0 : 1.0) : 1.0);
+      double ep = (mp > 0 && (mp & 1)) ? -1.0 : 1.0;
+      double f = em * ep;
+      double dm2 = 0.0, dm1 = d_start(mp, m, beta);
+      for (int n = l0; n < p; n++) {
+        double v;
+        if (n == l0)
+          v = dm1;
+        else if (n == 1)
+          v = cb; /* only m' = m = 0 reaches here */
+        else {
+          double a = (2.0 * n - 1.0) * (n * (n - 1.0) * cb - (double)mp * m);
+          double b = n * sqrt(fmax(0.0, ((n - 1.0) * (n - 1.0) - (double)mp * mp) *
+                                              ((n - 1.0) * (n - 1.0) - (double)m * m)));
+          double den = (n - 1.0) * sqrt(((double)n * n - (double)mp * mp) *
+                                        ((double)n * n - (double)m * m));
+          v = (a * dm1 - b * dm2) / den;
+        }
+        if (n > l0) {
+          dm2 = dm1;
+          dm1 = v;
+        }
+        H[n * (4 * n * n - 1) / 3 + (m + n) * (2 * n + 1) + (mp + n)] = f * v;
+      }
+    }
+}
+
+/* The reference's recurrence in m (sparseRotMatGen, translation.cu:1171-1237)
+ * restated in FP64.  It loses about one digit per degree beyond n ~ 15 (block
+ * orthogonality 4e-6 at p = 20, O(1) at p = 30), so it serves only as the
+ * small-p parity check of fo_rot_H. */
+void fo_rot_H_ref(int p, double beta, double *H) {
   int nmax = 2 * p - 2;
   if (nmax < 0) return;
   double cb = cos(beta), sb = sin(beta);
@@ -162,12 +230,17 @@ void fo_coax_sparse(int kind, int p, double k, double t, double *re,
                                 an;
       }
     }
-  /* sparse store (translation.cu:760-769) */
+  /* sparse store (translation.cu:760-769).  Only the entries with n' >= n
+   * are taken from the recurrence: for n' < n the advancing step subtracts
+   * terms (kt)^-2 larger than its result and loses accuracy when kt is
+   * small.  They follow from the exact symmetry
+   * (E|F)^m_{n'n} = (-1)^{n+n'} (E|F)^m_{nn'} of coaxial coefficients. */
   for (int m = 0; m < p; m++) {
     int off = fo_coax_block(p, m), ld = p - m;
     for (int np = m; np < p; np++)
       for (int n = m; n < p; n++) {
-        fo_cplx v = E[EIDX(m, np, n)];
+        fo_cplx v = np >= n ? E[EIDX(m, np, n)]
+                            : (((n + np) & 1) ? -1.0 : 1.0) * E[EIDX(m, n, np)];
         re[off + (n - m) * ld + (np - m)] = creal(v);
         im[off + (n - m) * ld + (np - m)] = cimag(v);
       }

@@ -214,11 +214,14 @@ int fo_tree_build(const double *V, int nv, const int *F, int nf, int s,
   }
   t->lmax = L;
   kv_t *kv = (kv_t *)malloc(sizeof(kv_t) * nf);
+  /* elements ordered by their level-10 key, then by id (a refinement of the
+   * leaf order used for the element permutation) */
   for (int i = 0; i < nf; i++) {
-    kv[i].key = fo_morton_key(ctr + 3 * i, L);
+    kv[i].key = fo_morton_key(ctr + 3 * i, 10);
     kv[i].idx = i;
   }
   qsort(kv, nf, sizeof(kv_t), cmp_kv);
+  for (int i = 0; i < nf; i++) kv[i].key >>= 3 * (10 - L);
   t->perm = (int *)malloc(sizeof(int) * nf);
   t->elem_box = (int *)malloc(sizeof(int) * nf);
   int nb = 0;

@@ -43,7 +43,7 @@ def test_tree_matches_oracle(sphere_l3):
     np.testing.assert_array_equal(perm, ot.perm())
 
 
-@pytest.mark.parametrize("p", [8, 17, 40])
+@pytest.mark.parametrize("p", [8, 17, 40, 94])
 def test_rot_mats_match_oracle(p):
     rng = np.random.default_rng(p)
     ang = np.column_stack([rng.uniform(-np.pi, np.pi, 5), rng.uniform(0, np.pi, 5),
@@ -55,14 +55,29 @@ def test_rot_mats_match_oracle(p):
         assert err < 1e-11, (i, err)
 
 
+def _coax_cols(mat, p):
+    """|m|-blocks of a sparse coaxial matrix as a list of (p-m)x(p-m) arrays [row n', col n]."""
+    out = []
+    for m in range(p):
+        off = O.lib().fo_coax_block(p, m)
+        ld = p - m
+        out.append(mat[off:off + ld * ld].reshape(ld, ld).T)
+    return out
+
+
 @pytest.mark.parametrize("kind,p,k,t", [(O.COAX_RR, 12, 18.3, 0.02), (O.COAX_SR, 16, 18.3, 0.1),
-                                        (O.COAX_SR, 30, 91.6, 0.3), (O.COAX_RR, 30, 91.6, 0.05)])
+                                        (O.COAX_SR, 30, 91.6, 0.3), (O.COAX_RR, 30, 91.6, 0.05),
+                                        (O.COAX_SR, 94, 183.2, 0.5), (O.COAX_RR, 94, 183.2, 0.108)])
 def test_coax_mats_match_oracle(kind, p, k, t):
+    """Entries agree to 1e-10 of the largest entry of their column (the
+    columns span many decades; the translation uses whole columns)."""
     got = F.gen_coax_mats(kind, k, [t, 1.5 * t], p)
     for i, tt in enumerate([t, 1.5 * t]):
         want = O.coax_sparse(kind, p, k, tt)
-        err = np.max(np.abs(got[i] - want) / np.maximum(np.abs(want), 1e-300 + np.abs(want).max() * 1e-14))
-        assert err < 1e-9, (i, err)
+        for bg, bw in zip(_coax_cols(got[i], p), _coax_cols(want, p)):
+            scale = np.abs(bw).max(axis=0)
+            err = np.max(np.abs(bg - bw) / scale[None, :])
+            assert err < 1e-10, (i, err)
 
 
 def _config1_pair(sphere_l3, precision=F.FMB_FP64, nrhs=1):
