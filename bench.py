@@ -1,0 +1,212 @@
+"""bench.py -- FMM matvecs/s and time to a 1e-6 GMRES residual, mixed vs FP64,
+on one MI355X.
+
+A step is one FMM-BEM matvec (Burton-Miller operator: near field + P2M, M2M,
+M2L, L2L, L2P) over all right-hand sides of the workload; value = right-hand-
+side matvecs per second of the mixed (FP32 pass) path with the inputs already
+in HBM.  The FP64 path is timed the same way, then both solve to a 1e-6 true
+(FP64) residual with GMRES(30).
+
+Workloads (synthetic geometry generated in-process):
+  sphere-l7-10k : icosphere level 7 (327,680 triangles), r = 0.5 m, 10 kHz,
+                  depth 7, truncNum orders, plane wave + 8 extra directions
+  sphere-l6-5k  : icosphere level 6 (81,920 triangles), r = 0.3 m, 5 kHz,
+                  depth 6, truncNum orders, one seeded plane-wave direction
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C_AIR = 343.0
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector / matrix dense
+
+WORKLOADS = {
+    "sphere-l7-10k": dict(level=7, radius=0.5, freq=10000.0, depth=7, nrhs=9),
+    "sphere-l6-5k": dict(level=6, radius=0.3, freq=5000.0, depth=6, nrhs=1),
+}
+
+
+def fibonacci_dirs(n, seed=0):
+    if n == 1:
+        v = np.random.default_rng(seed).standard_normal(3)
+        return (v / np.linalg.norm(v))[None, :]
+    i = np.arange(n) + 0.5
+    phi = np.arccos(1 - 2 * i / n)
+    th = np.pi * (1 + 5 ** 0.5) * i
+    d = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    d[0] = [0.0, 0.0, 1.0]
+    return d
+
+
+def m2l_flops(tree, plan, nrhs):
+    """RCR algorithmic flops per level: P_l * 8 (10 p^3 - p) / 3 per RHS."""
+    p = plan.orders()
+    _, L, _, _ = tree.info()
+    out = {}
+    for l in range(2, L + 1):
+        P = len(tree.pairs(l)[0])
+        out[l] = P * 8.0 * (10.0 * p[l] ** 3 - p[l]) / 3.0 * nrhs
+    return out
+
+
+def timed_matvecs(torch, plan, x, y, prec, steps, warmup, flops):
+    import fmbem_gpu_c_amd as F
+    for _ in range(warmup):
+        plan.matvec(x, y, prec=prec)
+    torch.cuda.synchronize()
+    plan.set_timing(True)
+    m2l_ms = {l: 0.0 for l in flops}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.matvec(x, y, prec=prec)
+        _, m2l = plan.timing()
+        for l in flops:
+            m2l_ms[l] += m2l[l]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.set_timing(False)
+    return dt, {l: v / steps for l, v in m2l_ms.items()}
+
+
+def cpu_baseline(V, F_, k, depth, seconds_cap):
+    """The FP64 oracle FMM (C + OpenMP) on the host cores: one matvec of the
+    same workload, or of a smaller icosphere when the full one would exceed
+    the time cap."""
+    import oracle as O
+    n = F_.shape[0]
+    ob = O.Bem(V, F_, k)
+    t0 = time.perf_counter()
+    ob.fmm_setup(s=5, depth=depth)
+    setup = time.perf_counter() - t0
+    x = np.random.default_rng(0).standard_normal(n) + 0j
+    t0 = time.perf_counter()
+    ob.fmm_matvec(x)
+    dt = time.perf_counter() - t0
+    return 1.0 / dt, O.lib().fo_num_threads(), setup, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sphere-l6-5k", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the FMM-BEM path here is single-GPU "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fmbem_gpu_c_amd as F
+
+    w = WORKLOADS[a.workload]
+    log = (lambda *s: print(*s, file=sys.stderr, flush=True)) if a.verbose else (lambda *s: None)
+    k = 2 * np.pi * w["freq"] / C_AIR
+    t0 = time.perf_counter()
+    mesh = F.Mesh.icosphere(w["level"], w["radius"])
+    V, Fc = mesh.arrays()
+    tree = F.Tree(mesh, depth=w["depth"])
+    plan = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=w["nrhs"])
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log("setup %.2fs orders %s mem %.2f GB" % (setup_s, plan.orders()[2:w["depth"] + 1],
+                                              plan.memory() / 1e9))
+    n = Fc.shape[0]
+    nrhs = w["nrhs"]
+    dirs = fibonacci_dirs(nrhs, seed=5)
+    b = plan.rhs_plane_wave(dirs)
+    x = torch.randn(nrhs, n, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+    flops = m2l_flops(tree, plan, nrhs)
+
+    dt32, m2l32 = timed_matvecs(torch, plan, x, y, F.FMB_MIXED, a.steps, a.warmup, flops)
+    dt64, m2l64 = timed_matvecs(torch, plan, x, y, F.FMB_FP64, a.steps, a.warmup, flops)
+    mv32 = a.steps * nrhs / dt32
+    mv64 = a.steps * nrhs / dt64
+    log("mixed %.3f ms/step, fp64 %.3f ms/step" % (1e3 * dt32 / a.steps, 1e3 * dt64 / a.steps))
+
+    # dominant kernel: the M2L launch of the level with the largest time
+    lbig = max(m2l32, key=lambda l: m2l32[l])
+    ach32 = flops[lbig] / (m2l32[lbig] * 1e-3) / 1e12
+    ach64 = flops[lbig] / (m2l64[lbig] * 1e-3) / 1e12
+
+    solve = {}
+    if not a.no_solve:
+        for name, prec in (("fp64", F.FMB_FP64), ("mixed", F.FMB_MIXED)):
+            plan.set_work_precision(prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xs, info = plan.gmres(b, restart=30, tol=1e-6, maxit=3000, check=False)
+            torch.cuda.synchronize()
+            solve[name] = dict(seconds=time.perf_counter() - t0, iters=info.iters,
+                               fp64_matvecs=info.matvecs_fp64, work_matvecs=info.matvecs_work,
+                               relres=max(info.relres[r] for r in range(nrhs)), x=xs)
+            log("solve %s %.2fs iters %d relres %.2e" % (name, solve[name]["seconds"],
+                                                        info.iters, solve[name]["relres"]))
+        u64, u32 = solve["fp64"]["x"].cpu().numpy(), solve["mixed"]["x"].cpu().numpy()
+        err_mixed = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+        import oracle as O
+        tri = V[Fc]
+        c = tri.mean(1)
+        d0 = dirs[0]
+        ct = (c @ d0) / np.linalg.norm(c, axis=1)
+        mie = O.mie_rigid(k * w["radius"], ct, nterms=int(k * w["radius"]) + 40)
+        err_mie = float(np.linalg.norm(u64[0] - mie) / np.linalg.norm(mie))
+        for s in solve.values():
+            s.pop("x")
+
+    cpu = None
+    if not a.no_cpu:
+        val, cores, csetup, cdt = cpu_baseline(V, Fc, k, w["depth"], 30.0)
+        cpu = {"value": val, "unit": "matvecs/s", "cores": cores, "kind": "port",
+               "sample": "one FP64 FMM matvec of the same %s mesh and tree by the C/OpenMP "
+                         "oracle (setup %.1fs excluded, matvec %.1fs)" % (a.workload, csetup, cdt)}
+
+    out = {
+        "metric": "FMM matvecs/s + time-to-1e-6 GMRES solve, mixed vs FP64, 1 GPU",
+        "value": mv32,
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt32 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (mixed: FP32 translations/near field, FP64 recurrences and Arnoldi)",
+        "data": "synthetic icosphere and plane waves generated in-process",
+        "config": {"workload": a.workload, "triangles": int(n), "freq_hz": w["freq"],
+                   "radius_m": w["radius"], "depth": w["depth"], "nrhs": nrhs,
+                   "orders": [int(q) for q in plan.orders()[2:w["depth"] + 1]],
+                   "parallelism": "1 GPU"},
+        "fp64_matvecs_per_s": mv64,
+        "speedup_mixed_vs_fp64_matvec": mv32 / mv64,
+        "setup_s": setup_s,
+        "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK["fp32"],
+                     "unit": "TFLOP/s", "frac": ach32 / PEAK["fp32"], "traffic": None,
+                     "kernel": "k_m2l level %d (FP32)" % lbig,
+                     "fp64_achieved": ach64, "fp64_frac": ach64 / PEAK["fp64"],
+                     "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
+                     "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},
+        "cpu_baseline": cpu,
+    }
+    if solve:
+        out["solve"] = {kk: {q: v for q, v in s.items()} for kk, s in solve.items()}
+        out["solve"]["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
+        out["solve"]["surface_pressure_rel_err_mixed_vs_fp64"] = err_mixed
+        out["solve"]["surface_pressure_rel_err_fp64_vs_mie"] = err_mie
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -101,6 +101,7 @@ _SIGS = {
     "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.POINTER(GmresInfo), _vp]),
+    "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fmb_plan_destroy": (None, [_vp]),
     "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
     "fmb_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -316,6 +317,9 @@ class Plan:
         if check:
             _check(rc)
         return x, info
+
+    def set_work_precision(self, prec):
+        _check(lib().fmb_plan_set_work_precision(self.h, prec))
 
     def set_timing(self, on=True):
         _check(lib().fmb_set_timing(self.h, int(on)))

@@ -95,6 +95,7 @@ struct fmb_plan {
   int *d_tab_mmajor[FMB_MAXLEV]; /* per level: m-major (n',m) enumeration   */
   fmb_opset ops[2];     /* [0] FP64, [1] FP32 (FMB_MIXED only)            */
   int nops;
+  int work_prec;        /* precision of the Arnoldi matvecs in fmb_gmres   */
   int nrhs_max;
   double *d_work64;     /* [nrhs][N] FP64 complex scratch                  */
   int timing;

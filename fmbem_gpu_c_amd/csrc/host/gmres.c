@@ -38,7 +38,7 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
   double t0 = now_s();
   const int N = pl->tree->nelem, m = restart;
-  const int work = pl->nops == 2 ? FMB_MIXED : FMB_FP64;
+  const int work = pl->work_prec;
   size_t vbytes = sizeof(fmb_c64) * (size_t)N * nrhs;
   int rc;
   fmb_c64 *dV = NULL, *dw = NULL, *dr = NULL, *dh = NULL, *dh2 = NULL, *dy = NULL;

@@ -374,6 +374,7 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     }
   }
   pl->nops = prm->precision == FMB_MIXED ? 2 : 1;
+  pl->work_prec = prm->precision;
   for (int i = 0; i < pl->nops; i++) {
     if ((rc = opset_alloc(pl, &pl->ops[i], i == 0 ? FMB_FP64 : FMB_MIXED))) goto fail;
     if ((rc = build_opset(pl, &pl->ops[i], d_H64, d_sr64, d_rr64, d_robin)))
@@ -407,6 +408,14 @@ int fmb_plan_orders(const fmb_plan *pl, int *p16) {
 int fmb_plan_memory(const fmb_plan *pl, double *bytes) {
   if (!pl || !bytes) return fmb_set_error(FMB_ERR_ARG, "null plan");
   *bytes = (double)pl->bytes;
+  return FMB_OK;
+}
+
+int fmb_plan_set_work_precision(fmb_plan *pl, int prec) {
+  if (!pl || (prec != FMB_FP64 && prec != FMB_MIXED) ||
+      (prec == FMB_MIXED && pl->nops < 2))
+    return fmb_set_error(FMB_ERR_ARG, "working precision %d not built in this plan", prec);
+  pl->work_prec = prec;
   return FMB_OK;
 }
 

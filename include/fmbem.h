@@ -188,6 +188,9 @@ typedef struct fmb_gmres_info {
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream);
+/* Precision of the Arnoldi matvecs of fmb_gmres on this plan (FMB_MIXED only
+ * on a plan created with FMB_MIXED).  The reference has no solver. */
+int fmb_plan_set_work_precision(fmb_plan *pl, int prec);
 void fmb_plan_destroy(fmb_plan *pl);
 
 /* Per-phase timing of the last fmb_matvec (ms): near, p2m, m2m, m2l, l2l,
