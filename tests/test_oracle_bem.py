@@ -1,0 +1,99 @@
+"""Oracle Burton-Miller BEM (fo_bem.c): the FMM matvec against the dense
+O(N^2) matvec with the same quadrature, the dense solve against the rigid-
+and impedance-sphere Mie series (scipy), the oracle's Mie series against
+scipy golden values, and GMRES against numpy.linalg.solve."""
+import os
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import fmbem_gpu_c_amd as F
+import oracle as O
+from conftest import GOLDEN, wavenumber
+
+G = np.load(os.path.join(GOLDEN, "special.npz"))
+
+
+@pytest.fixture(scope="module")
+def sphere2():
+    return F.Mesh.icosphere(2, 0.1).arrays()
+
+
+@pytest.fixture(scope="module")
+def sphere3():
+    return F.Mesh.icosphere(3, 0.1).arrays()
+
+
+def test_mie_matches_scipy_golden():
+    for ka, key in ((G["mie_ka1"], "mie1"), (G["mie_ka2"], "mie2")):
+        got = O.mie_rigid(float(ka), G["mie_ct"], nterms=40 if key == "mie1" else 70)
+        np.testing.assert_allclose(got, G[key], rtol=1e-10)
+
+
+@pytest.mark.parametrize("plev,tol", [({2: 16, 3: 14}, 3e-6), ({2: 22, 3: 20}, 1e-7)])
+def test_fmm_matches_dense(sphere3, plev, tol):
+    V, Fc = sphere3
+    k = wavenumber(1000.0)
+    b = O.Bem(V, Fc, k)
+    b.fmm_setup(depth=3, plev=plev)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(b.n) + 1j * rng.standard_normal(b.n)
+    yd = b.dense_matvec(x)[0]
+    yf = b.fmm_matvec(x)[0]
+    assert np.linalg.norm(yf - yd) / np.linalg.norm(yd) < tol
+
+
+def test_fmm_error_decreases_with_order(sphere3):
+    V, Fc = sphere3
+    k = wavenumber(1000.0)
+    x = np.random.default_rng(2).standard_normal(len(Fc)) + 0j
+    yd = O.Bem(V, Fc, k).dense_matvec(x)[0]
+    errs = []
+    for p in (10, 14, 18, 22):
+        b = O.Bem(V, Fc, k)
+        b.fmm_setup(depth=3, plev={2: p, 3: p})
+        errs.append(np.linalg.norm(b.fmm_matvec(x)[0] - yd) / np.linalg.norm(yd))
+    assert all(a > 1.5 * c for a, c in zip(errs, errs[1:])), errs
+
+
+def _mie_impedance(ka, k, a, sigma, ct, nterms):
+    """Surface pressure on a sphere with dp/dr = sigma p, plane wave e^{ikz}:
+    sum (2n+1) i^n P_n (i / (k a^2)) / (k h_n'(ka) - sigma h_n(ka))."""
+    n = np.arange(nterms + 1)
+    h = sp.spherical_jn(n, ka) + 1j * sp.spherical_yn(n, ka)
+    hp = sp.spherical_jn(n, ka, True) + 1j * sp.spherical_yn(n, ka, True)
+    c = (2 * n + 1) * 1j ** n * (1j / (k * a * a)) / (k * hp - sigma * h)
+    P = np.array([sp.eval_legendre(q, ct) for q in n])
+    return (c[:, None] * P).sum(0)
+
+
+@pytest.mark.parametrize("Y", [0.0, 0.3 + 0.2j])
+def test_dense_solution_converges_to_mie(sphere2, sphere3, Y):
+    errs = []
+    for V, Fc in (sphere2, sphere3):
+        k = wavenumber(1000.0)
+        sig = 1j * k * Y * np.ones(len(Fc))
+        b = O.Bem(V, Fc, k, sigma=sig if Y else None)
+        A = b.dense_matrix()
+        u = np.linalg.solve(A, b.rhs_plane([0, 0, 1.0]))
+        c, _, _ = b.centroids()
+        ct = c[:, 2] / np.linalg.norm(c, axis=1)
+        ref = _mie_impedance(k * 0.1, k, 0.1, 1j * k * Y, ct, 40)
+        errs.append(np.linalg.norm(u - ref) / np.linalg.norm(ref))
+    # rigid: 1.5% -> 0.5%; impedance: 3.4% -> 1.5% (first order in h)
+    assert errs[1] < (0.01 if Y == 0 else 0.02) and errs[1] < 0.6 * errs[0], errs
+
+
+def test_gmres_matches_direct_solve(sphere2):
+    V, Fc = sphere2
+    k = wavenumber(1500.0)
+    b = O.Bem(V, Fc, k)
+    A = b.dense_matrix()
+    rhs = b.rhs_plane([0.6, 0.0, 0.8])
+    u = np.linalg.solve(A, rhs)
+    x, it, rr, rc = b.gmres(rhs, use_fmm=False, m=10, tol=1e-10)
+    assert rc == 0 and rr <= 1e-10
+    assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-8
+    # restarts happened (m=10) and the true residual is what was reported
+    assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) == pytest.approx(rr, rel=1e-3)
