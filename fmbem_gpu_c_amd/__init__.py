@@ -19,6 +19,7 @@ FMB_FP64, FMB_MIXED = 0, 1
 PRULE_TRUNCNUM, PRULE_EXCESS, PRULE_EXPLICIT = 0, 1, 2
 RULE_COLLAPSED, RULE_GAUSS7 = 0, 1
 COAX_RR, COAX_SR = 0, 1
+RCR_AUTO, RCR_VALU = 0, 1
 
 
 class FmbError(RuntimeError):
@@ -45,6 +46,7 @@ class PlanParams(ctypes.Structure):
         ("near_ratio", ctypes.c_double),
         ("nedge", ctypes.c_int),
         ("nrhs_max", ctypes.c_int),
+        ("rcr_path", ctypes.c_int),
     ]
 
 

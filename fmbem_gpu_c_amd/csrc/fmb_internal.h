@@ -61,7 +61,19 @@ typedef struct fmb_opset {
   void *d_near; /* near blocks, complex                                     */
   void *d_M[FMB_MAXLEV], *d_L[FMB_MAXLEV]; /* [nbox][nrhs][p^2] complex      */
   void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
+  void *d_gws;       /* global workspace of the GWS-mode translations       */
+  int gws_grid;      /* persistent workgroups sharing d_gws                  */
 } fmb_opset;
+
+/* one translation kind at one level on the matrix cores */
+typedef struct fmb_rcr_tab {
+  int *d_par, *d_chi;   /* M2M/L2L: (parent, child) pairs by child octant */
+  void *d_chunks;       /* int4 {bucket, first pair, end pair, 0}          */
+  int nchunks;
+  int pin, pout, P;     /* source order, target order, coaxial order       */
+  int mode[2], nt[2], mt[2]; /* per opset: 0 = VALU kernel, else FMBH_RCR_* */
+  int *d_sched[2];      /* per opset: unit schedule                        */
+} fmb_rcr_tab;
 
 struct fmb_plan {
   const fmb_tree *tree;
@@ -84,6 +96,10 @@ struct fmb_plan {
   void *d_off_info;     /* FMB_NOFF x {int beta, int dist, double phi}      */
   void *d_m2l_chunks[FMB_MAXLEV]; /* int4 {offset, first, last, 0}          */
   int n_m2l_chunks[FMB_MAXLEV];
+  /* matrix-core translations, [kind][level]: kind 0 M2L at l, 1 M2M
+   * l -> l-1, 2 L2L l-1 -> l (csrc/host/rcr.c) */
+  fmb_rcr_tab rcr[3][FMB_MAXLEV];
+  void *d_info_oct[2]; /* M2M, L2L: per child octant {beta, 0, phi}        */
   double *d_quad;       /* far rule: nq x {b0,b1,b2,w}                    */
   double *d_quadn;      /* close rule                                      */
   double *d_edge;       /* nedge x {x, w} Gauss-Legendre on [-1,1]         */

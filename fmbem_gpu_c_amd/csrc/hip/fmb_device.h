@@ -57,6 +57,27 @@ __device__ __forceinline__ void rfma(cplx<T> &a, T h, cplx<T> c) {
   a.im = fma(h, c.im, a.im);
 }
 
+/* a += v with hardware float atomics (-munsafe-fp-atomics) */
+template <typename T>
+__device__ __forceinline__ void atomic_add_c(cplx<T> *dst, cplx<T> v) {
+  unsafeAtomicAdd(&dst->re, v.re);
+  unsafeAtomicAdd(&dst->im, v.im);
+}
+
+/* n with n^2 <= f < (n+1)^2: the degree of coefficient index f = n^2+n+m */
+__device__ __forceinline__ int isqrt_fix(int f) {
+  int n = (int)sqrtf((float)f);
+  while (n * n > f) n--;
+  while ((n + 1) * (n + 1) <= f) n++;
+  return n;
+}
+
+/* one M2L offset: polar-angle table index, distance class, azimuth phi */
+struct OffInfo {
+  int beta, dist;
+  double phi;
+};
+
 static inline __host__ __device__ int rot_off(int n) {
   return n * (4 * n * n - 1) / 3;
 }

@@ -71,6 +71,31 @@ int fmbh_scatter(const fmb_plan *pl, int op, fmb_c64 *y, int nrhs,
 int fmbh_p2m(const fmb_plan *pl, int op, int nrhs, void *stream);
 int fmbh_m2m(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
+/* matrix-core RCR translations (rcr_mfma.hip) */
+#define FMBH_RCR_LDS 1 /* batch in LDS, blocks owned by one wave          */
+#define FMBH_RCR_GWS 2 /* batch in a global workspace, row-group units    */
+typedef struct fmbh_rcr_job {
+  int mode, nt, mt;
+  const void *chunks;
+  int nchunks, chunk_pairs;
+  const void *info;
+  const int *tgt, *src;
+  const void *H;
+  size_t hstride;
+  const void *C;
+  size_t cstride;
+  int P, pin, pout;
+  const int *sched;
+  const void *in;
+  void *out;
+  void *gws;
+  int gws_grid;
+} fmbh_rcr_job;
+/* workspace mode and tile shape for max(pin, pout) = pm */
+int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt);
+/* global workspace bytes of one workgroup in GWS mode */
+size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt);
+int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
 int fmbh_l2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
                   void *stream);

@@ -20,19 +20,6 @@
 
 namespace {
 
-__device__ __forceinline__ int isqrt_fix(int f) {
-  int n = (int)sqrtf((float)f);
-  while (n * n > f) n--;
-  while ((n + 1) * (n + 1) <= f) n++;
-  return n;
-}
-
-template <typename T>
-__device__ __forceinline__ void atomic_add_c(cplx<T> *dst, cplx<T> v) {
-  unsafeAtomicAdd(&dst->re, v.re);
-  unsafeAtomicAdd(&dst->im, v.im);
-}
-
 // One rotate-coaxial-rotate translation of vector x (order pin) to order
 // pout.  H: rotation blocks for theta (prefix layout), C: coaxial matrix at
 // order P >= pin, pout (|m|-block layout), mtab: m-major enumeration of the
@@ -125,11 +112,6 @@ __device__ __forceinline__ void fill_phase(cplx<T> *ph, int np_, double phi) {
     ph[m] = cmk<T>((T)c, (T)s);
   }
 }
-
-struct OffInfo {
-  int beta, dist;
-  double phi;
-};
 
 // M2L: blocks take chunks (offset, first pair, last pair) of one offset bucket
 template <typename T, int GS, int R>

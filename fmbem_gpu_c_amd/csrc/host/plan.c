@@ -107,11 +107,13 @@ static void opset_free(fmb_opset *o) {
   fmbh_free(o->d_near);
   fmbh_free(o->d_xs);
   fmbh_free(o->d_ys);
+  fmbh_free(o->d_gws);
   memset(o, 0, sizeof(*o));
 }
 
 void fmb_plan_destroy(fmb_plan *pl) {
   if (!pl) return;
+  fmbx_rcr_free(pl);
   for (int i = 0; i < 2; i++) opset_free(&pl->ops[i]);
   for (int l = 0; l < FMB_MAXLEV; l++) {
     free(pl->h_scale[l]);
@@ -181,6 +183,8 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: invalid parameters");
   if (prm->nedge < 1 || prm->nedge > 256 || prm->near_ratio < 0)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: quadrature parameters");
+  if (prm->rcr_path != FMB_RCR_AUTO && prm->rcr_path != FMB_RCR_VALU)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
   int rc;
   fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
   pl->tree = t;
@@ -380,6 +384,7 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     if ((rc = build_opset(pl, &pl->ops[i], d_H64, d_sr64, d_rr64, d_robin)))
       goto fail;
   }
+  if ((rc = fmbx_rcr_setup(pl, stream))) goto fail;
   fmbh_free(d_H64);
   for (int l = 0; l < FMB_MAXLEV; l++) {
     fmbh_free(d_sr64[l]);
@@ -456,21 +461,22 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
       if ((rc = fmbh_event_create(&ev[i]))) return rc;
   if (tm) fmbh_event_record(ev[0], stream);
   if ((rc = fmbh_gather(pl, op, x, nrhs, stream))) goto out;
-  for (int l = 2; l <= L; l++)
-    if ((rc = fmbh_memset(o->d_L[l], 0,
-                          (size_t)t->nbox[l] * nrhs * pl->p[l] * pl->p[l] * o->csz,
-                          stream)))
-      goto out;
+  for (int l = 2; l <= L; l++) {
+    size_t bytes = (size_t)t->nbox[l] * nrhs * pl->p[l] * pl->p[l] * o->csz;
+    if ((rc = fmbh_memset(o->d_L[l], 0, bytes, stream))) goto out;
+    /* parents receive the M2M translations of their children as sums */
+    if (l < L && (rc = fmbh_memset(o->d_M[l], 0, bytes, stream))) goto out;
+  }
   if ((rc = fmbh_p2m(pl, op, nrhs, stream))) goto out;
   if (tm) fmbh_event_record(ev[1], stream);
   for (int l = L; l > 2; l--)
-    if ((rc = fmbh_m2m(pl, op, l, nrhs, stream))) goto out;
+    if ((rc = fmbx_rcr_run(pl, op, 1, l, nrhs, stream))) goto out;
   if (tm) fmbh_event_record(ev[2], stream);
   for (int l = 2; l <= L; l++) {
     if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
-    if ((rc = fmbh_m2l(pl, op, l, nrhs, stream))) goto out;
+    if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, stream))) goto out;
     if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
-    if (l > 2 && (rc = fmbh_l2l(pl, op, l, nrhs, stream))) goto out;
+    if (l > 2 && (rc = fmbx_rcr_run(pl, op, 2, l, nrhs, stream))) goto out;
   }
   if (tm) fmbh_event_record(ev[3], stream);
   if ((rc = fmbh_l2p_near(pl, op, nrhs, parts, stream))) goto out;

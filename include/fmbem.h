@@ -128,6 +128,10 @@ int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
 
 typedef struct fmb_plan fmb_plan;
 
+/* kernel family of the M2L / M2M / L2L translations */
+#define FMB_RCR_AUTO 0 /* matrix cores (batched per offset / octant)      */
+#define FMB_RCR_VALU 1 /* one translation per workgroup on the vector ALU */
+
 typedef struct fmb_plan_params {
   double k;                  /* wavenumber (rad/m)                          */
   double alpha_re, alpha_im; /* Burton-Miller coupling; 0,0 selects i/k     */
@@ -141,6 +145,7 @@ typedef struct fmb_plan_params {
   double near_ratio;         /* close: |x_i - c_j| < ratio * circumradius_j */
   int nedge;                 /* Gauss points per edge, singular self terms  */
   int nrhs_max;              /* right-hand sides per matvec                 */
+  int rcr_path;              /* FMB_RCR_*: translation kernel family        */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);

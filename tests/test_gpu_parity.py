@@ -66,18 +66,36 @@ def _coax_cols(mat, p):
 
 
 @pytest.mark.parametrize("kind,p,k,t", [(O.COAX_RR, 12, 18.3, 0.02), (O.COAX_SR, 16, 18.3, 0.1),
-                                        (O.COAX_SR, 30, 91.6, 0.3), (O.COAX_RR, 30, 91.6, 0.05),
-                                        (O.COAX_SR, 94, 183.2, 0.5), (O.COAX_RR, 94, 183.2, 0.108)])
+                                        (O.COAX_SR, 30, 91.6, 0.3), (O.COAX_RR, 30, 91.6, 0.05)])
 def test_coax_mats_match_oracle(kind, p, k, t):
-    """Entries agree to 1e-10 of the largest entry of their column (the
-    columns span many decades; the translation uses whole columns)."""
+    """Entries agree to 1e-8 of the largest entry of their column (the
+    columns span many decades; the translation uses whole columns).  Both
+    sides run the same FP64 recurrences, the device with fused multiply-adds
+    and the host without; the sectorial and advancing recurrences amplify
+    those last-bit differences (a few 1e-9 at kt = 41, p = 30), still far
+    below the FP32 pass's rounding."""
     got = F.gen_coax_mats(kind, k, [t, 1.5 * t], p)
     for i, tt in enumerate([t, 1.5 * t]):
         want = O.coax_sparse(kind, p, k, tt)
+        err = 0.0
         for bg, bw in zip(_coax_cols(got[i], p), _coax_cols(want, p)):
             scale = np.abs(bw).max(axis=0)
-            err = np.max(np.abs(bg - bw) / scale[None, :])
-            assert err < 1e-10, (i, err)
+            err = max(err, float(np.max(np.abs(bg - bw) / scale[None, :])))
+        assert err < 1e-8, (i, tt, err)
+
+
+def test_coax_mats_vs_50_digit_values():
+    """Device (R|R), (S|R) matrices up to p = 94 against the recurrences run
+    in 50-digit arithmetic (tests/golden/coax_mp.npz), with the oracle's
+    bounds: 1e-10 of the column maximum for |m| <= p/2, 1e-6 towards
+    |m| = p-1 where the FP64 sectorial recurrence itself loses digits."""
+    from test_oracle_ops import _mp_cases, coax_mp_error
+    for tag, kind, p, k, t, blocks in _mp_cases():
+        got = F.gen_coax_mats(kind, k, [t], p)[0]
+        lo, hi = coax_mp_error(got, p, blocks)
+        print("coax %s: |m|<=p/2 %.2e, tail %.2e" % (tag, lo, hi), flush=True)
+        assert lo < 1e-10, (tag, lo)
+        assert hi < 1e-6, (tag, hi)
 
 
 def _config1_pair(sphere_l3, precision=F.FMB_FP64, nrhs=1):
@@ -132,3 +150,41 @@ def test_gmres_config1(sphere_l3, precision):
     ct = c[:, 2] / np.linalg.norm(c, axis=1)
     mie = O.mie_rigid(k * 0.1, ct)
     assert np.linalg.norm(u - mie) / np.linalg.norm(mie) < 0.02
+
+
+def _far_field(pl, x, prec):
+    return pl.matvec_parts(x, 2, prec=prec).cpu().numpy()
+
+
+@pytest.mark.parametrize("freq,orders", [(4000.0, [0, 0, 6, 12, 20, 17]),
+                                         (4000.0, [0, 0, 33, 28, 24, 8]),
+                                         (4000.0, [0, 0, 47, 40, 31, 25]),
+                                         (16000.0, [0, 0, 62, 44, 30, 20])])
+def test_rcr_matrix_cores_match_valu(freq, orders):
+    """The matrix-core translations (rcr_mfma.hip: M2L, M2M, L2L) against the
+    one-translation-per-workgroup VALU kernels on the same plan: far field
+    only (P2M .. L2P), three right-hand sides so batches straddle box pairs,
+    level orders rising and falling so M2M / L2L run with pin != pout, and
+    orders chosen to run every LDS tile shape and the global-workspace mode
+    of both precisions.  FP64: 1e-12 relative (summation order only); FP32
+    pass: 1e-5 relative to the FP32 VALU pass on the same operators."""
+    m = F.Mesh.icosphere(4, 0.1)
+    k = wavenumber(freq)
+    t = F.Tree(m, depth=5)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=3, p_rule=F.PRULE_EXPLICIT, p_levels=orders)
+    pv = F.Plan(t, k, rcr_path=F.RCR_VALU, **kw)
+    pm = F.Plan(t, k, rcr_path=F.RCR_AUTO, **kw)
+    rng = np.random.default_rng(11)
+    n = m.arrays()[1].shape[0]
+    x = torch.tensor(rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n)), device="cuda")
+    yv = _far_field(pv, x, F.FMB_FP64)
+    ym = _far_field(pm, x, F.FMB_FP64)
+    e64 = np.linalg.norm(ym - yv) / np.linalg.norm(yv)
+    yv32 = _far_field(pv, x, F.FMB_MIXED)
+    ym32 = _far_field(pm, x, F.FMB_MIXED)
+    e32 = np.linalg.norm(ym32 - yv32) / np.linalg.norm(yv32)
+    e32_64 = np.linalg.norm(ym32 - yv) / np.linalg.norm(yv)
+    print("f %g orders %s: fp64 %.2e fp32 %.2e (vs fp64 %.2e)"
+          % (freq, orders[2:], e64, e32, e32_64), flush=True)
+    assert e64 < 1e-12
+    assert e32 < 1e-5
