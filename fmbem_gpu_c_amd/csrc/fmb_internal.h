@@ -62,6 +62,8 @@ typedef struct fmb_opset {
   void *d_M[FMB_MAXLEV], *d_L[FMB_MAXLEV]; /* [nbox][nrhs][p^2] complex      */
   void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
   void *d_gws;       /* global workspace of the GWS-mode translations       */
+  void *d_Hpad;      /* zero-padded rotation blocks [nbeta][hpad_stride]    */
+  size_t hpad_stride;
   int gws_grid;      /* persistent workgroups sharing d_gws                  */
 } fmb_opset;
 
@@ -73,6 +75,9 @@ typedef struct fmb_rcr_tab {
   int pin, pout, P;     /* source order, target order, coaxial order       */
   int mode[2], nt[2], mt[2]; /* per opset: 0 = VALU kernel, else FMBH_RCR_* */
   int *d_sched[2];      /* per opset: unit schedule                        */
+  int sched_len[2];     /* ints in d_sched                                 */
+  void *d_Cpad[2];      /* per opset: zero-padded coaxial blocks           */
+  size_t cpad_stride;   /* complex values per coaxial matrix in d_Cpad     */
 } fmb_rcr_tab;
 
 struct fmb_plan {

@@ -86,16 +86,26 @@ typedef struct fmbh_rcr_job {
   size_t cstride;
   int P, pin, pout;
   const int *sched;
+  int sched_len;
   const void *in;
   void *out;
   void *gws;
   int gws_grid;
 } fmbh_rcr_job;
-/* workspace mode and tile shape for max(pin, pout) = pm */
+/* workspace mode and tile shape for max(pin, pout) = pm (the LDS budget
+ * leaves room for a schedule of up to 8 pm + 16 ints) */
 int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt);
 /* global workspace bytes of one workgroup in GWS mode */
 size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt);
 int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
+/* zero-padded operator copies: rotation block n at hoff[n] (16-row x
+ * 4-column multiples, leading dimension = padded rows); coaxial block |m|
+ * (rows pout-|m|, columns pin-|m|) at coff[|m|] */
+int fmbh_pad_rot(int prec, const void *d_H, int nbeta, int Pmax,
+                 const int *d_hoff, size_t pstride, void *d_Hp, void *stream);
+int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pin,
+                  int pout, const int *d_coff, size_t pstride, void *d_Cp,
+                  void *stream);
 int fmbh_l2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
                   void *stream);

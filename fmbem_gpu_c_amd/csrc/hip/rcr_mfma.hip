@@ -20,6 +20,11 @@
 //     v_mfma_{f32,f64}_16x16x4 (A = H from global memory, B = X rows);
 //   * coaxial block m: X_m <- C_m X_m, complex, as Cr*B + Ci*B' with
 //     B' = (-im, re) formed from the same rows.
+// The operator blocks are stored zero-padded to 16-row / 4-column multiples
+// (column-major, leading dimension = padded rows), so operand loads need no
+// guards: rows of X beyond a block are read at a clamped index and meet
+// zero operator entries.  Operands of the next KU K-steps are loaded into one
+// register set while the MFMAs consume the other.
 // Two workspace modes:
 //   LDS: X in LDS, updated in place; each block is owned by one wave (LPT
 //        schedule built on the host), which keeps the block's outputs in its
@@ -55,147 +60,230 @@ template <> struct Mfma<double> {
   static __device__ __forceinline__ int row(int q, int e) { return q + 4 * e; }
 };
 
+// Values the wave shares (schedule entries, offsets read from LDS) are
+// moved to scalar registers so that the branches and addresses derived from
+// them stay scalar.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <int NT> struct Row {
   static constexpr int stride = 16 * NT + 2;  // reals per X row
 };
+
+// K-steps per operand register set
+template <typename T, int NTL> struct Ahead {
+  static constexpr int ku = (sizeof(T) == 4 || NTL <= 2) ? 4 : 2;
+};
+
+__device__ __forceinline__ int pad16(int s) { return (s + 15) & ~15; }
 
 struct RcrArgs {
   const int4 *chunks;  // {bucket, first pair, end pair, 0}
   int nchunks, nb;     // chunks, batches per chunk (grid y / item stride)
   const OffInfo *info; // per bucket: beta index, coaxial matrix index, phi
   const int *ptgt, *psrc;
-  const void *H;       // [nbeta][rot_off(Pmax)]
+  const void *H;       // padded rotation blocks, per beta
   size_t hstride;
-  const void *C;       // [ncoax][coax_size(P)]
+  const void *C;       // padded coaxial blocks, per coaxial matrix
   size_t cstride;
   int P, pin, pout;
-  const int *sched;    // {w1, wc, w3}, [4][w1], [4][wc], [4][w3] unit codes
+  const int *sched;    // offsets [3][kWaves + 1], then {code, operator offset}
+  int sched_len;
   const void *Min;     // [box][nrhs][pin^2]
   void *Lout;          // [box][nrhs][pout^2], added to
   int nrhs;
   void *gws;           // GWS mode: per-workgroup 2 x Pm^2 x RS reals
 };
 
-// Rotation stage over this wave's units n*16 + g (-1 terminated): row tiles
-// [g*MT, g*MT + MT) of block n, read from Xi, written to Xo.
-template <typename T, int NT, int MT>
-__device__ __forceinline__ void rot_stage(const T *Xi, T *Xo,
-                                          const T *__restrict__ H,
-                                          const int *__restrict__ units,
-                                          int width) {
+// Row tiles [t0, t0 + NTL) of rotation block n (size s, first row base):
+// Xo[rows] = Hn Xi[rows].  Ab: the block (padded, leading dimension pad16(s)).
+template <typename T, int NT, int NTL>
+__device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
+                                         const T *__restrict__ Ab, int s,
+                                         int base, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
-  constexpr int RS = Row<NT>::stride;
+  constexpr int RS = Row<NT>::stride, KU = Ahead<T, NTL>::ku;
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  for (int j = 0; j < width; j++) {
-    const int code = units[j];
-    if (code < 0) break;
-    const int n = code >> 4, t0 = (code & 15) * MT;
-    const int s = 2 * n + 1, base = n * n;
-    const int nt = min(MT, ((s + 15) >> 4) - t0), nks = (s + 3) >> 2;
-    const T *Hn = H + rot_off(n);
-    acc_t acc[MT][NT];
+  const int ld = pad16(s), nks = (s + 3) >> 2, ng = (nks + KU - 1) / KU;
+  const T *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  acc_t acc[NTL][NT];
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++)
+  for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-      for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
-    for (int ks = 0; ks < nks; ks++) {
-      const int k = 4 * ks + lk;
-      const bool kin = k < s;
-      T b[NT];
+    for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
+  T a0[KU][NTL], b0[KU][NT], a1[KU][NTL], b1[KU][NT];
+  auto fetch = [&](int g, T(&fa)[KU][NTL], T(&fb)[KU][NT]) {
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        b[t] = kin ? Xi[(base + k) * RS + 16 * t + li] : T(0);
-      const T *hk = Hn + (size_t)k * s;
+    for (int u = 0; u < KU; u++) {
+      const int ks = min(g * KU + u, nks - 1);
+      const int k = min(4 * ks + lk, s - 1);
+      const T *xr = Xi + (base + k) * RS + li;
 #pragma unroll
-      for (int mt = 0; mt < MT; mt++) {
-        if (mt < nt) {
-          const int i = 16 * (t0 + mt) + li;
-          const T a = (kin && i < s) ? hk[i] : T(0);
+      for (int t = 0; t < NT; t++) fb[u][t] = xr[16 * t];
+      const T *ak = ap + (unsigned)(4 * ks * ld);
 #pragma unroll
-          for (int t = 0; t < NT; t++) acc[mt][t] = Mfma<T>::mma(a, b[t], acc[mt][t]);
-        }
+      for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
+    }
+  };
+  auto mma = [&](int g, const T(&fa)[KU][NTL], const T(&fb)[KU][NT]) {
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      if (g * KU + u < nks) {
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+          for (int t = 0; t < NT; t++) acc[mt][t] = Mfma<T>::mma(fa[u][mt], fb[u][t], acc[mt][t]);
       }
     }
+  };
+  fetch(0, a0, b0);
+  for (int g = 0; g < ng; g += 2) {
+    if (g + 1 < ng) fetch(g + 1, a1, b1);
+    mma(g, a0, b0);
+    if (g + 1 < ng) {
+      if (g + 2 < ng) fetch(g + 2, a0, b0);
+      mma(g + 1, a1, b1);
+    }
+  }
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-      if (mt < nt) {
+  for (int mt = 0; mt < NTL; mt++) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
-          if (i < s) {
+    for (int e = 0; e < 4; e++) {
+      const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
+      if (i < s) {
 #pragma unroll
-            for (int t = 0; t < NT; t++) Xo[(base + i) * RS + 16 * t + li] = acc[mt][t][e];
-          }
-        }
+        for (int t = 0; t < NT; t++) Xo[(base + i) * RS + 16 * t + li] = acc[mt][t][e];
       }
     }
   }
 }
 
-// Coaxial stage over this wave's units (m + P)*16 + g: block m maps rows
-// (n, m), n = |m| .. pin-1, to rows (n', m), n' = |m| .. pout-1.
-template <typename T, int NT, int MT>
-__device__ __forceinline__ void coax_stage(const T *Xi, T *Xo,
-                                           const cplx<T> *__restrict__ C,
-                                           const int *__restrict__ units,
-                                           int width, int P, int pin, int pout) {
+// Row tiles [t0, t0 + NTL) of coaxial block m: rows (n', m), n' = |m| ..
+// pout-1, from rows (n, m), n = |m| .. pin-1.  Ab: the block (padded,
+// leading dimension pad16(so)).
+template <typename T, int NT, int NTL>
+__device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
+                                          const cplx<T> *__restrict__ Ab,
+                                          int m, int pin, int pout, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
-  constexpr int RS = Row<NT>::stride;
+  constexpr int RS = Row<NT>::stride, KU = Ahead<T, NTL>::ku;
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   const bool odd = li & 1;
-  for (int j = 0; j < width; j++) {
-    const int code = units[j];
-    if (code < 0) break;
-    const int m = (code >> 4) - P, t0 = (code & 15) * MT;
-    const int am = m < 0 ? -m : m, so = pout - am, si = pin - am, ld = P - am;
-    const int nt = min(MT, ((so + 15) >> 4) - t0), nks = (si + 3) >> 2;
-    const cplx<T> *Cm = C + coax_blk(P, am);
-    acc_t acc[MT][NT];
+  const int am = m < 0 ? -m : m, so = pout - am, si = pin - am;
+  const int ld = pad16(so), nks = (si + 3) >> 2, ng = (nks + KU - 1) / KU;
+  const cplx<T> *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  acc_t acc[NTL][NT];
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++)
+  for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-      for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
-    for (int ks = 0; ks < nks; ks++) {
-      const int k = 4 * ks + lk;
-      const bool kin = k < si;
-      const int nn = am + k, row = nn * nn + nn + m;
-      T b[NT], bq[NT];
+    for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
+  cplx<T> a0[KU][NTL], z0[KU][NT], a1[KU][NTL], z1[KU][NT];
+  auto fetch = [&](int g, cplx<T>(&fa)[KU][NTL], cplx<T>(&fz)[KU][NT]) {
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        cplx<T> z = cmk<T>(0, 0);
-        if (kin) z = *(const cplx<T> *)(Xi + row * RS + 16 * t + (li & ~1));
-        b[t] = odd ? z.im : z.re;
-        bq[t] = odd ? z.re : -z.im;
-      }
-      const cplx<T> *ck = Cm + (size_t)k * ld;
+    for (int u = 0; u < KU; u++) {
+      const int ks = min(g * KU + u, nks - 1);
+      const int nn = am + min(4 * ks + lk, si - 1), row = nn * nn + nn + m;
+      const T *xr = Xi + row * RS + (li & ~1);
 #pragma unroll
-      for (int mt = 0; mt < MT; mt++) {
-        if (mt < nt) {
-          const int i = 16 * (t0 + mt) + li;
-          cplx<T> a = cmk<T>(0, 0);
-          if (kin && i < so) a = ck[i];
+      for (int t = 0; t < NT; t++) fz[u][t] = *(const cplx<T> *)(xr + 16 * t);
+      const cplx<T> *ak = ap + (unsigned)(4 * ks * ld);
+#pragma unroll
+      for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
+    }
+  };
+  auto mma = [&](int g, const cplx<T>(&fa)[KU][NTL], const cplx<T>(&fz)[KU][NT]) {
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      if (g * KU + u < nks) {
+        T b[NT], bq[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          b[t] = odd ? fz[u][t].im : fz[u][t].re;
+          bq[t] = odd ? fz[u][t].re : -fz[u][t].im;
+        }
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
           for (int t = 0; t < NT; t++) {
-            acc[mt][t] = Mfma<T>::mma(a.re, b[t], acc[mt][t]);
-            acc[mt][t] = Mfma<T>::mma(a.im, bq[t], acc[mt][t]);
+            acc[mt][t] = Mfma<T>::mma(fa[u][mt].re, b[t], acc[mt][t]);
+            acc[mt][t] = Mfma<T>::mma(fa[u][mt].im, bq[t], acc[mt][t]);
           }
-        }
       }
     }
+  };
+  fetch(0, a0, z0);
+  for (int g = 0; g < ng; g += 2) {
+    if (g + 1 < ng) fetch(g + 1, a1, z1);
+    mma(g, a0, z0);
+    if (g + 1 < ng) {
+      if (g + 2 < ng) fetch(g + 2, a0, z0);
+      mma(g + 1, a1, z1);
+    }
+  }
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-      if (mt < nt) {
+  for (int mt = 0; mt < NTL; mt++) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
-          if (i < so) {
-            const int nn = am + i, row = nn * nn + nn + m;
+    for (int e = 0; e < 4; e++) {
+      const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
+      if (i < so) {
+        const int nn = am + i, row = nn * nn + nn + m;
 #pragma unroll
-            for (int t = 0; t < NT; t++) Xo[row * RS + 16 * t + li] = acc[mt][t][e];
-          }
-        }
+        for (int t = 0; t < NT; t++) Xo[row * RS + 16 * t + li] = acc[mt][t][e];
       }
     }
+  }
+}
+
+// runs unit<NTL = ntl> for a wave-uniform ntl in [1, MT]
+template <typename T, int NT, int MT>
+__device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo,
+                                        const T *Ab, int s, int base, int t0) {
+  if constexpr (MT > 1) {
+    if (ntl < MT) {
+      rot_any<T, NT, MT - 1>(ntl, Xi, Xo, Ab, s, base, t0);
+      return;
+    }
+  }
+  rot_unit<T, NT, MT>(Xi, Xo, Ab, s, base, t0);
+}
+
+template <typename T, int NT, int MT>
+__device__ __forceinline__ void coax_any(int ntl, const T *Xi, T *Xo,
+                                         const cplx<T> *Ab, int m, int pin,
+                                         int pout, int t0) {
+  if constexpr (MT > 1) {
+    if (ntl < MT) {
+      coax_any<T, NT, MT - 1>(ntl, Xi, Xo, Ab, m, pin, pout, t0);
+      return;
+    }
+  }
+  coax_unit<T, NT, MT>(Xi, Xo, Ab, m, pin, pout, t0);
+}
+
+// Rotation stage over this wave's units {n*16 + g, operator offset}: row
+// tiles [g*MT, g*MT + MT) of block n.
+template <typename T, int NT, int MT>
+__device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
+                                          const int *units, int count) {
+  for (int j = 0; j < count; j++) {
+    const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
+    const int n = code >> 4, t0 = (code & 15) * MT;
+    const int s = 2 * n + 1;
+    const int ntl = min(MT, ((s + 15) >> 4) - t0);
+    rot_any<T, NT, MT>(ntl, Xi, Xo, H + aoff, s, n * n, t0);
+  }
+}
+
+// Coaxial stage over this wave's units {(m + P)*16 + g, operator offset}.
+template <typename T, int NT, int MT>
+__device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
+                                           const int *units, int count, int P,
+                                           int pin, int pout) {
+  for (int j = 0; j < count; j++) {
+    const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
+    const int m = (code >> 4) - P, t0 = (code & 15) * MT;
+    const int so = pout - (m < 0 ? -m : m);
+    const int ntl = min(MT, ((so + 15) >> 4) - t0);
+    coax_any<T, NT, MT>(ntl, Xi, Xo, C + aoff, m, pin, pout, t0);
   }
 }
 
@@ -203,9 +291,9 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo,
 template <typename T, int NT, int MT, bool GWS>
 __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
                                           T *X0, T *X1, cplx<T> *ph,
-                                          long long *vsrc, long long *vdst) {
+                                          const int *sched) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = uni(tid >> 6);
   const int pin = a.pin, pout = a.pout, P = a.P;
   const int Pm = pin > pout ? pin : pout;
   const int Pi2 = pin * pin, Po2 = pout * pout, Pm2 = Pm * Pm;
@@ -219,57 +307,56 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
     sincos(m * oi.phi, &sn, &cs);
     ph[m] = cmk<T>((T)cs, (T)sn);
   }
-  if (tid < V) {
-    const int u = u0 + tid;
-    long long so = -1, d = -1;
+  // vector v of the batch: (pair, right-hand side) = divmod(u0 + v, nrhs);
+  // its source and target base pointers are wave-uniform
+  const cplx<T> *vin[V];
+  cplx<T> *vout[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const int u = u0 + v;
+    vin[v] = nullptr;
+    vout[v] = nullptr;
     if (u < U) {
       const int q = ch.y + u / a.nrhs, r = u % a.nrhs;
-      so = ((long long)a.psrc[q] * a.nrhs + r) * Pi2;
-      d = ((long long)a.ptgt[q] * a.nrhs + r) * Po2;
+      vin[v] = (const cplx<T> *)a.Min + ((size_t)a.psrc[q] * a.nrhs + r) * Pi2;
+      vout[v] = (cplx<T> *)a.Lout + ((size_t)a.ptgt[q] * a.nrhs + r) * Po2;
     }
-    vsrc[tid] = so;
-    vdst[tid] = d;
   }
   __syncthreads();
   // source coefficients times e^{i m phi}; rows f >= pin^2 and absent
-  // vectors are zero
-  const cplx<T> *Min = (const cplx<T> *)a.Min;
-  for (int v = 0; v < V; v++) {
-    const long long so = vsrc[v];
-    for (int f = tid; f < Pm2; f += kThreads) {
-      cplx<T> z = cmk<T>(0, 0);
-      if (so >= 0 && f < Pi2) {
-        const int n = isqrt_fix(f), m = f - n * n - n;
-        cplx<T> w = ph[m < 0 ? -m : m];
-        if (m < 0) w.im = -w.im;
-        z = cmul(Min[so + f], w);
-      }
-      *(cplx<T> *)(X0 + f * RS + 2 * v) = z;
-    }
+  // vectors are zero.  One thread per row f: its V loads are in flight
+  // together, the degree and phase of f are computed once.
+  for (int f = tid; f < Pm2; f += kThreads) {
+    cplx<T> z[V];
+#pragma unroll
+    for (int v = 0; v < V; v++)
+      z[v] = (vin[v] && f < Pi2) ? vin[v][(unsigned)f] : cmk<T>(0, 0);
+    const int n = isqrt_fix(f), m = f - n * n - n;
+    cplx<T> w = ph[m < 0 ? -m : m];
+    if (m < 0) w.im = -w.im;
+#pragma unroll
+    for (int v = 0; v < V; v++) *(cplx<T> *)(X0 + f * RS + 2 * v) = cmul(z[v], w);
   }
   __syncthreads();
-  const int *sc = a.sched;
-  const int w1 = sc[0], wc = sc[1], w3 = sc[2];
-  const int *u1 = sc + 3 + wave * w1;
-  const int *uc = sc + 3 + kWaves * w1 + wave * wc;
-  const int *u3 = sc + 3 + kWaves * (w1 + wc) + wave * w3;
+  const int *o1 = sched, *oc = sched + kWaves + 1, *o3 = sched + 2 * (kWaves + 1);
+  const int *u1 = sched + 2 * uni(o1[wave]), w1 = uni(o1[wave + 1] - o1[wave]);
+  const int *uc = sched + 2 * uni(oc[wave]), wc = uni(oc[wave + 1] - oc[wave]);
+  const int *u3 = sched + 2 * uni(o3[wave]), w3 = uni(o3[wave + 1] - o3[wave]);
   rot_stage<T, NT, MT>(X0, X1, H, u1, w1);
   __syncthreads();
   coax_stage<T, NT, MT>(X1, X0, C, uc, wc, P, pin, pout);
   __syncthreads();
   rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
   __syncthreads();
-  cplx<T> *Lout = (cplx<T> *)a.Lout;
-  for (int v = 0; v < V; v++) {
-    const long long d = vdst[v];
-    if (d < 0) continue;
-    for (int f = tid; f < Po2; f += kThreads) {
-      const int n = isqrt_fix(f), m = f - n * n - n;
-      cplx<T> w = ph[m < 0 ? -m : m];
-      if (m > 0) w.im = -w.im;
-      const cplx<T> z = *(const cplx<T> *)(X1 + f * RS + 2 * v);
-      atomic_add_c(Lout + d + f, cmul(z, w));
-    }
+  // add e^{-i m' phi} y into the targets
+  for (int f = tid; f < Po2; f += kThreads) {
+    const int n = isqrt_fix(f), m = f - n * n - n;
+    cplx<T> w = ph[m < 0 ? -m : m];
+    if (m > 0) w.im = -w.im;
+#pragma unroll
+    for (int v = 0; v < V; v++)
+      if (vout[v])
+        atomic_add_c(vout[v] + (unsigned)f, cmul(*(const cplx<T> *)(X1 + f * RS + 2 * v), w));
   }
 }
 
@@ -284,8 +371,9 @@ __global__ void __launch_bounds__(kThreads) k_rcr_lds(RcrArgs a) {
   if (u0 >= (ch.z - ch.y) * a.nrhs) return;
   T *X = (T *)smem;
   cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * RS);
-  long long *vsrc = (long long *)(ph + Pm + (Pm & 1));
-  rcr_batch<T, NT, MT, false>(a, ch, u0, X, X, ph, vsrc, vsrc + V);
+  int *sched = (int *)(ph + Pm + (Pm & 1));
+  for (int i = threadIdx.x; i < a.sched_len; i += kThreads) sched[i] = a.sched[i];
+  rcr_batch<T, NT, MT, false>(a, ch, u0, X, X, ph, sched);
 }
 
 // GWS mode: persistent grid over items chunk * nb + batch.
@@ -298,13 +386,43 @@ __global__ void __launch_bounds__(kThreads) k_rcr_gws(RcrArgs a) {
   T *X0 = (T *)a.gws + 2 * slice * blockIdx.x;
   T *X1 = X0 + slice;
   cplx<T> *ph = (cplx<T> *)smem;
-  long long *vsrc = (long long *)(ph + Pm + (Pm & 1));
+  int *sched = (int *)(ph + Pm + (Pm & 1));
+  for (int i = threadIdx.x; i < a.sched_len; i += kThreads) sched[i] = a.sched[i];
   const long long items = (long long)a.nchunks * a.nb;
   for (long long it = blockIdx.x; it < items; it += gridDim.x) {
     const int c = (int)(it / a.nb), b = (int)(it % a.nb);
     const int4 ch = a.chunks[c];
     if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
-    rcr_batch<T, NT, MT, true>(a, ch, b * V, X0, X1, ph, vsrc, vsrc + V);
+    rcr_batch<T, NT, MT, true>(a, ch, b * V, X0, X1, ph, sched);
+  }
+}
+
+// zero-padded copies of the operator blocks (see the file comment)
+template <typename T>
+__global__ void k_pad_rot(const T *__restrict__ H, size_t hs, const int *__restrict__ hoff,
+                          size_t ps, T *__restrict__ Hp) {
+  const int n = blockIdx.x, b = blockIdx.y;
+  const int s = 2 * n + 1, ld = (s + 15) & ~15, nc = (s + 3) & ~3;
+  const T *src = H + hs * b + rot_off(n);
+  T *dst = Hp + ps * b + hoff[n];
+  for (int e = threadIdx.x; e < ld * nc; e += blockDim.x) {
+    const int i = e % ld, k = e / ld;
+    dst[e] = (i < s && k < s) ? src[k * s + i] : T(0);
+  }
+}
+
+template <typename T>
+__global__ void k_pad_coax(const cplx<T> *__restrict__ C, size_t cs, int P, int pin,
+                           int pout, const int *__restrict__ coff, size_t ps,
+                           cplx<T> *__restrict__ Cp) {
+  const int am = blockIdx.x, d = blockIdx.y;
+  const int so = pout - am, si = pin - am, ldc = P - am;
+  const int ld = (so + 15) & ~15, nc = (si + 3) & ~3;
+  const cplx<T> *src = C + cs * d + coax_blk(P, am);
+  cplx<T> *dst = Cp + ps * d + coff[am];
+  for (int e = threadIdx.x; e < ld * nc; e += blockDim.x) {
+    const int i = e % ld, k = e / ld;
+    dst[e] = (i < so && k < si) ? src[k * ldc + i] : cmk<T>(0, 0);
   }
 }
 
@@ -317,8 +435,9 @@ int mt_lds(int pm) {
   return 0;
 }
 
-size_t small_lds(int pm, int nt, size_t tsz) {
-  return (size_t)(pm + (pm & 1)) * 2 * tsz + 2 * 8 * nt * sizeof(long long);
+// phase table and a schedule of sched_len ints
+size_t small_lds(int pm, size_t tsz, int sched_len) {
+  return (size_t)(pm + (pm & 1)) * 2 * tsz + sizeof(int) * (size_t)sched_len;
 }
 
 size_t x_bytes(int pm, int nt, size_t tsz) {
@@ -353,20 +472,23 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   a.pin = j->pin;
   a.pout = j->pout;
   a.sched = j->sched;
+  a.sched_len = j->sched_len;
   a.Min = j->in;
   a.Lout = j->out;
   a.nrhs = nrhs;
   a.gws = j->gws;
   const size_t tsz = sizeof(T);
   if (j->mode == FMBH_RCR_LDS) {
-    const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, NT, tsz);
+    const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_lds<T, NT, MT>;
     int rc = set_lds(kern, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(kern, dim3(a.nchunks, a.nb), dim3(kThreads), lds, st, a);
   } else {
-    const size_t lds = small_lds(Pm, NT, tsz);
+    const size_t lds = small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_gws<T, NT, MT>;
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
     long long items = (long long)a.nchunks * a.nb;
     int grid = items < j->gws_grid ? (int)items : j->gws_grid;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, st, a);
@@ -389,11 +511,12 @@ int dispatch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
 
 extern "C" int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt) {
   const size_t tsz = prec == FMB_FP64 ? sizeof(double) : sizeof(float);
+  const int slen = 8 * pm + 16; /* whole-block schedule bound */
   *mt = mt_lds(pm);
-  if (*mt && x_bytes(pm, 2, tsz) + small_lds(pm, 2, tsz) <= 80 * 1024) {
+  if (*mt && x_bytes(pm, 2, tsz) + small_lds(pm, tsz, slen) <= 80 * 1024) {
     *mode = FMBH_RCR_LDS;
     *nt = 2;
-  } else if (*mt && x_bytes(pm, 1, tsz) + small_lds(pm, 1, tsz) <= 160 * 1024) {
+  } else if (*mt && x_bytes(pm, 1, tsz) + small_lds(pm, tsz, slen) <= 160 * 1024) {
     *mode = FMBH_RCR_LDS;
     *nt = 1;
   } else {
@@ -413,4 +536,37 @@ extern "C" int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream)
   if (j->nchunks == 0) return FMB_OK;
   if (prec == FMB_FP64) return dispatch<double>(j, nrhs, (hipStream_t)stream);
   return dispatch<float>(j, nrhs, (hipStream_t)stream);
+}
+
+extern "C" int fmbh_pad_rot(int prec, const void *d_H, int nbeta, int Pmax,
+                            const int *d_hoff, size_t pstride, void *d_Hp,
+                            void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  size_t hs = (size_t)rot_off(Pmax);
+  if (prec == FMB_FP64)
+    hipLaunchKernelGGL(k_pad_rot<double>, dim3(Pmax, nbeta), dim3(256), 0, st,
+                       (const double *)d_H, hs, d_hoff, pstride, (double *)d_Hp);
+  else
+    hipLaunchKernelGGL(k_pad_rot<float>, dim3(Pmax, nbeta), dim3(256), 0, st,
+                       (const float *)d_H, hs, d_hoff, pstride, (float *)d_Hp);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+extern "C" int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pin,
+                             int pout, const int *d_coff, size_t pstride,
+                             void *d_Cp, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int mn = pin < pout ? pin : pout;
+  size_t cs = (size_t)coax_size(P);
+  if (prec == FMB_FP64)
+    hipLaunchKernelGGL(k_pad_coax<double>, dim3(mn, ncoax), dim3(256), 0, st,
+                       (const cplx<double> *)d_C, cs, P, pin, pout, d_coff, pstride,
+                       (cplx<double> *)d_Cp);
+  else
+    hipLaunchKernelGGL(k_pad_coax<float>, dim3(mn, ncoax), dim3(256), 0, st,
+                       (const cplx<float> *)d_C, cs, P, pin, pout, d_coff, pstride,
+                       (cplx<float> *)d_Cp);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
 }

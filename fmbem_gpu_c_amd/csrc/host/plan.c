@@ -108,6 +108,7 @@ static void opset_free(fmb_opset *o) {
   fmbh_free(o->d_xs);
   fmbh_free(o->d_ys);
   fmbh_free(o->d_gws);
+  fmbh_free(o->d_Hpad);
   memset(o, 0, sizeof(*o));
 }
 

@@ -30,7 +30,7 @@ typedef struct {
 } rcr_info_t; /* layout of the device OffInfo */
 
 typedef struct {
-  int code, cost;
+  int code, cost, aoff; /* aoff: offset of the unit's padded operator block */
 } unit_t;
 
 static int by_cost(const void *a, const void *b) {
@@ -40,18 +40,31 @@ static int by_cost(const void *a, const void *b) {
 }
 
 /* units in non-increasing cost order, each to the least-loaded wave;
- * out[w * n + i], -1 padded */
-static void lpt(unit_t *u, int n, int *out) {
+ * writes the waves' lists back to back as {code, aoff} pairs into out
+ * (entry q at out[2q]) and their start entries (RCR_WAVES + 1 values,
+ * beginning at base) into off */
+static void lpt(unit_t *u, int n, int base, int *off, int *out) {
   int load[RCR_WAVES] = {0}, cnt[RCR_WAVES] = {0};
+  int *w_of = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   qsort(u, n, sizeof(unit_t), by_cost);
-  for (int i = 0; i < RCR_WAVES * n; i++) out[i] = -1;
   for (int i = 0; i < n; i++) {
     int w = 0;
     for (int q = 1; q < RCR_WAVES; q++)
       if (load[q] < load[w]) w = q;
     load[w] += u[i].cost;
-    out[w * n + cnt[w]++] = u[i].code;
+    w_of[i] = w;
+    cnt[w]++;
   }
+  off[0] = base;
+  for (int w = 0; w < RCR_WAVES; w++) off[w + 1] = off[w] + cnt[w];
+  int fill[RCR_WAVES];
+  for (int w = 0; w < RCR_WAVES; w++) fill[w] = off[w];
+  for (int i = 0; i < n; i++) {
+    int q = fill[w_of[i]]++;
+    out[2 * q] = u[i].code;
+    out[2 * q + 1] = u[i].aoff;
+  }
+  free(w_of);
 }
 
 static int tiles(int s) { return (s + 15) / 16; }
@@ -61,14 +74,33 @@ static int group_tiles(int nt, int g, int mt, int split) {
   return nt - g * mt < mt ? nt - g * mt : mt;
 }
 
+/* padded block sizes: rows to a multiple of 16, columns to a multiple of 4 */
+static int pad_rows(int s) { return 16 * tiles(s); }
+static int pad_cols(int s) { return 4 * ksteps(s); }
+
+/* offsets of the padded rotation blocks n < np (np + 1 values) */
+static void rot_offsets(int np, int *hoff) {
+  hoff[0] = 0;
+  for (int n = 0; n < np; n++) hoff[n + 1] = hoff[n] + pad_rows(2 * n + 1) * pad_cols(2 * n + 1);
+}
+
+/* offsets of the padded coaxial blocks |m| < min(pin, pout) (+1 values) */
+static void coax_offsets(int pin, int pout, int *coff) {
+  int mn = pin < pout ? pin : pout;
+  coff[0] = 0;
+  for (int am = 0; am < mn; am++)
+    coff[am + 1] = coff[am] + pad_rows(pout - am) * pad_cols(pin - am);
+}
+
 /* rotation blocks n < np: code n*16 + g */
-static int rot_units(int np, int split, int mt, unit_t *u) {
+static int rot_units(int np, int split, int mt, const int *hoff, unit_t *u) {
   int k = 0;
   for (int n = 0; n < np; n++) {
     int s = 2 * n + 1, nt = tiles(s), ng = split ? (nt + mt - 1) / mt : 1;
     for (int g = 0; g < ng; g++) {
       u[k].code = n * 16 + g;
       u[k].cost = group_tiles(nt, g, mt, split) * ksteps(s);
+      u[k].aoff = hoff[n];
       k++;
     }
   }
@@ -76,7 +108,8 @@ static int rot_units(int np, int split, int mt, unit_t *u) {
 }
 
 /* coaxial blocks |m| < min(pin, pout): code (m + P)*16 + g */
-static int coax_units(int pin, int pout, int P, int split, int mt, unit_t *u) {
+static int coax_units(int pin, int pout, int P, int split, int mt, const int *coff,
+                      unit_t *u) {
   int k = 0, mn = pin < pout ? pin : pout;
   for (int m = -(mn - 1); m <= mn - 1; m++) {
     int am = m < 0 ? -m : m, so = pout - am, si = pin - am;
@@ -84,29 +117,34 @@ static int coax_units(int pin, int pout, int P, int split, int mt, unit_t *u) {
     for (int g = 0; g < ng; g++) {
       u[k].code = (m + P) * 16 + g;
       u[k].cost = 2 * group_tiles(nt, g, mt, split) * ksteps(si);
+      u[k].aoff = coff[am];
       k++;
     }
   }
   return k;
 }
 
-/* {w1, wc, w3} then [4][w1] forward rotation, [4][wc] coaxial,
- * [4][w3] back rotation */
-static int build_sched(int pin, int pout, int P, int mode, int mt, int **d_out,
-                       void *stream) {
+/* Schedule of one translation shape: a header of three offset tables
+ * (forward rotation, coaxial, back rotation), RCR_WAVES + 1 entry indices
+ * each, giving every wave's units as entries [off[w], off[w+1]); entry q is
+ * the int pair {code, operator offset} at [2q, 2q + 1] of the same array. */
+#define RCR_SCHED_HDR 16 /* 3 (RCR_WAVES + 1) ints, rounded up to even */
+static int build_sched(int pin, int pout, int P, int mode, int mt, const int *hoff,
+                       const int *coff, int **d_out, int *len, void *stream) {
   int split = mode == FMBH_RCR_GWS, rc;
   unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + 2 * P + pout));
-  int n1 = rot_units(pin, split, mt, u);
-  int nc = coax_units(pin, pout, P, split, mt, u + n1);
-  int n3 = rot_units(pout, split, mt, u + n1 + nc);
-  int tot = 3 + RCR_WAVES * (n1 + nc + n3);
-  int *h = (int *)malloc(sizeof(int) * tot);
-  h[0] = n1;
-  h[1] = nc;
-  h[2] = n3;
-  lpt(u, n1, h + 3);
-  lpt(u + n1, nc, h + 3 + RCR_WAVES * n1);
-  lpt(u + n1 + nc, n3, h + 3 + RCR_WAVES * (n1 + nc));
+  int n1 = rot_units(pin, split, mt, hoff, u);
+  int nc = coax_units(pin, pout, P, split, mt, coff, u + n1);
+  int n3 = rot_units(pout, split, mt, hoff, u + n1 + nc);
+  int tot = RCR_SCHED_HDR + 2 * (n1 + nc + n3);
+  int *h = (int *)calloc(tot, sizeof(int));
+  int base = RCR_SCHED_HDR / 2;
+  lpt(u, n1, base, h, h);
+  base += n1;
+  lpt(u + n1, nc, base, h + RCR_WAVES + 1, h);
+  base += nc;
+  lpt(u + n1 + nc, n3, base, h + 2 * (RCR_WAVES + 1), h);
+  *len = tot;
   rc = fmbh_malloc((void **)d_out, sizeof(int) * tot);
   if (!rc) rc = fmbh_h2d(*d_out, h, sizeof(int) * tot, stream);
   free(u);
@@ -166,22 +204,42 @@ static int octant_pairs(const fmb_tree *t, int l, fmb_rcr_tab *tb, void *stream)
   return rc;
 }
 
-static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout, void *stream) {
-  int rc;
+/* tile shapes, schedules and padded coaxial blocks of one translation;
+ * C: the scaled coaxial matrices of each opset (ncoax of them, order P) */
+static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
+                    void *const *C, int ncoax, const int *hoff, void *stream) {
+  int rc = FMB_OK, mn = pin < pout ? pin : pout;
   tb->pin = pin;
   tb->pout = pout;
   tb->P = pin > pout ? pin : pout;
-  for (int op = 0; op < pl->nops; op++) {
+  int *coff = (int *)malloc(sizeof(int) * (mn + 1));
+  int *d_coff = NULL;
+  coax_offsets(pin, pout, coff);
+  tb->cpad_stride = (size_t)coff[mn];
+  for (int op = 0; op < pl->nops && !rc; op++) {
+    fmb_opset *o = &pl->ops[op];
     tb->mode[op] = 0;
     if (pl->prm.rcr_path == FMB_RCR_VALU) continue;
-    if ((rc = fmbh_rcr_shape(tb->P, pl->ops[op].prec, &tb->mode[op], &tb->nt[op],
-                             &tb->mt[op])))
-      return rc;
-    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op],
-                          &tb->d_sched[op], stream)))
-      return rc;
+    if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
+      break;
+    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], hoff, coff,
+                          &tb->d_sched[op], &tb->sched_len[op], stream)))
+      break;
+    if (!d_coff) {
+      if ((rc = fmbh_malloc((void **)&d_coff, sizeof(int) * (mn + 1)))) break;
+      if ((rc = fmbh_h2d(d_coff, coff, sizeof(int) * (mn + 1), stream))) break;
+    }
+    size_t bytes = o->csz * tb->cpad_stride * ncoax;
+    if ((rc = fmbh_malloc(&tb->d_Cpad[op], bytes))) break;
+    pl->bytes += bytes;
+    if ((rc = fmbh_pad_coax(o->prec, C[op], ncoax, tb->P, pin, pout, d_coff,
+                            tb->cpad_stride, tb->d_Cpad[op], stream)))
+      break;
   }
-  return FMB_OK;
+  if (!rc) rc = fmbh_sync(stream);
+  fmbh_free(d_coff);
+  free(coff);
+  return rc;
 }
 
 int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
@@ -197,16 +255,41 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     if ((rc = fmbh_malloc(&pl->d_info_oct[kind - 1], sizeof(info)))) return rc;
     if ((rc = fmbh_h2d(pl->d_info_oct[kind - 1], info, sizeof(info), stream))) return rc;
   }
-  for (int l = 2; l <= L; l++) {
-    fmb_rcr_tab *tb = &pl->rcr[0][l];
-    if ((rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream))) return rc;
-    if ((rc = plan_tab(pl, tb, pl->p[l], pl->p[l], stream))) return rc;
-  }
-  for (int l = 3; l <= L; l++) {
-    fmb_rcr_tab *up = &pl->rcr[1][l], *dn = &pl->rcr[2][l];
-    if ((rc = octant_pairs(t, l, up, stream))) return rc;
-    if ((rc = plan_tab(pl, up, pl->p[l], pl->p[l - 1], stream))) return rc;
-    if ((rc = plan_tab(pl, dn, pl->p[l - 1], pl->p[l], stream))) return rc;
+  if (pl->prm.rcr_path != FMB_RCR_VALU) {
+    /* padded rotation blocks of every polar angle, per opset */
+    int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1)), *d_hoff = NULL;
+    rot_offsets(pl->Pmax, hoff);
+    rc = fmbh_malloc((void **)&d_hoff, sizeof(int) * (pl->Pmax + 1));
+    if (!rc) rc = fmbh_h2d(d_hoff, hoff, sizeof(int) * (pl->Pmax + 1), stream);
+    for (int op = 0; op < pl->nops && !rc; op++) {
+      fmb_opset *o = &pl->ops[op];
+      o->hpad_stride = (size_t)hoff[pl->Pmax];
+      size_t bytes = o->csz / 2 * o->hpad_stride * pl->nbeta;
+      rc = fmbh_malloc(&o->d_Hpad, bytes);
+      if (!rc) {
+        pl->bytes += bytes;
+        rc = fmbh_pad_rot(o->prec, o->d_H, pl->nbeta, pl->Pmax, d_hoff, o->hpad_stride,
+                          o->d_Hpad, stream);
+      }
+    }
+    if (!rc) rc = fmbh_sync(stream);
+    fmbh_free(d_hoff);
+    for (int l = 2; l <= L && !rc; l++) {
+      fmb_rcr_tab *tb = &pl->rcr[0][l];
+      void *C[2] = {pl->ops[0].d_Csr[l], pl->ops[1].d_Csr[l]};
+      if ((rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream))) break;
+      rc = plan_tab(pl, tb, pl->p[l], pl->p[l], C, FMB_NDIST, hoff, stream);
+    }
+    for (int l = 3; l <= L && !rc; l++) {
+      fmb_rcr_tab *up = &pl->rcr[1][l], *dn = &pl->rcr[2][l];
+      void *Cu[2] = {pl->ops[0].d_Cm2m[l], pl->ops[1].d_Cm2m[l]};
+      void *Cd[2] = {pl->ops[0].d_Cl2l[l], pl->ops[1].d_Cl2l[l]};
+      if ((rc = octant_pairs(t, l, up, stream))) break;
+      if ((rc = plan_tab(pl, up, pl->p[l], pl->p[l - 1], Cu, 1, hoff, stream))) break;
+      rc = plan_tab(pl, dn, pl->p[l - 1], pl->p[l], Cd, 1, hoff, stream);
+    }
+    free(hoff);
+    if (rc) return rc;
   }
   /* global workspace: the largest GWS batch of each opset */
   for (int op = 0; op < pl->nops; op++) {
@@ -233,7 +316,10 @@ void fmbx_rcr_free(fmb_plan *pl) {
       fmbh_free(tb->d_chunks);
       fmbh_free(tb->d_par);
       fmbh_free(tb->d_chi);
-      for (int op = 0; op < 2; op++) fmbh_free(tb->d_sched[op]);
+      for (int op = 0; op < 2; op++) {
+        fmbh_free(tb->d_sched[op]);
+        fmbh_free(tb->d_Cpad[op]);
+      }
       memset(tb, 0, sizeof(*tb));
     }
   for (int i = 0; i < 2; i++) {
@@ -262,34 +348,33 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.chunks = pairs->d_chunks;
   j.nchunks = pairs->nchunks;
   j.chunk_pairs = RCR_CHUNK;
-  j.H = o->d_H;
-  j.hstride = (size_t)fmb_rot_size(pl->Pmax);
+  j.H = o->d_Hpad;
+  j.hstride = o->hpad_stride;
+  j.C = tb->d_Cpad[op];
+  j.cstride = tb->cpad_stride;
   j.P = tb->P;
   j.pin = tb->pin;
   j.pout = tb->pout;
   j.sched = tb->d_sched[op];
+  j.sched_len = tb->sched_len[op];
   j.gws = o->d_gws;
   j.gws_grid = o->gws_grid;
   if (kind == 0) {
     j.info = pl->d_off_info;
     j.tgt = t->d_pair_tgt[l];
     j.src = t->d_pair_src[l];
-    j.C = o->d_Csr[l];
-    j.cstride = (size_t)fmb_coax_size(tb->P);
     j.in = o->d_M[l];
     j.out = o->d_L[l];
   } else if (kind == 1) {
     j.info = pl->d_info_oct[0];
     j.tgt = pairs->d_par;
     j.src = pairs->d_chi;
-    j.C = o->d_Cm2m[l];
     j.in = o->d_M[l];
     j.out = o->d_M[l - 1];
   } else {
     j.info = pl->d_info_oct[1];
     j.tgt = pairs->d_chi;
     j.src = pairs->d_par;
-    j.C = o->d_Cl2l[l];
     j.in = o->d_L[l - 1];
     j.out = o->d_L[l];
   }
