@@ -28,8 +28,8 @@ C_AIR = 343.0
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector / matrix dense
 
 WORKLOADS = {
-    "sphere-l7-10k": dict(level=7, radius=0.5, freq=10000.0, depth=7, nrhs=9),
-    "sphere-l6-5k": dict(level=6, radius=0.3, freq=5000.0, depth=6, nrhs=1),
+    "sphere-l7-10k": dict(level=7, radius=0.5, freq=10000.0, depth=7, nrhs=9, cpu_stride=8),
+    "sphere-l6-5k": dict(level=6, radius=0.3, freq=5000.0, depth=6, nrhs=1, cpu_stride=1),
 }
 
 
@@ -75,21 +75,37 @@ def timed_matvecs(torch, plan, x, y, prec, steps, warmup, flops):
     return dt, {l: v / steps for l, v in m2l_ms.items()}
 
 
-def cpu_baseline(V, F_, k, depth, seconds_cap):
-    """The FP64 oracle FMM (C + OpenMP) on the host cores: one matvec of the
-    same workload, or of a smaller icosphere when the full one would exceed
-    the time cap."""
+def cpu_baseline(V, F_, k, depth, stride):
+    """The FP64 oracle FMM (C + OpenMP) on the host cores for the same mesh,
+    tree and orders: P2M and M2M in full, M2L / L2L / L2P / near field on the
+    first 1/stride of the leaf boxes and their ancestors (levels too small to
+    sample run whole), each level's time scaled to all its boxes.  Returns
+    (matvecs/s for one right-hand side, threads, setup s, sample s, est s)."""
     import oracle as O
-    n = F_.shape[0]
     ob = O.Bem(V, F_, k)
     t0 = time.perf_counter()
     ob.fmm_setup(s=5, depth=depth)
     setup = time.perf_counter() - t0
-    x = np.random.default_rng(0).standard_normal(n) + 0j
     t0 = time.perf_counter()
-    ob.fmm_matvec(x)
-    dt = time.perf_counter() - t0
-    return 1.0 / dt, O.lib().fo_num_threads(), setup, dt
+    est, _ = ob.fmm_time_sample(stride)
+    sample = time.perf_counter() - t0
+    return 1.0 / est, O.lib().fo_num_threads(), setup, sample, est
+
+
+def traffic_of(workload, lbig):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 counter summary of this workload (FETCH_SIZE, WRITE_SIZE,
+    each scaled by what it reads for an elementwise kernel of known size),
+    or None when there is none for this level."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    lv = d.get("levels", {}).get(str(lbig))
+    if not lv:
+        return None
+    return lv["fetch_kb"] * 1024 * d["fetch_scale"] + lv["write_kb"] * 1024 * d["write_scale"]
 
 
 def main():
@@ -97,7 +113,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="sphere-l6-5k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sphere-l7-10k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -140,6 +156,11 @@ def main():
     mv32 = a.steps * nrhs / dt32
     mv64 = a.steps * nrhs / dt64
     log("mixed %.3f ms/step, fp64 %.3f ms/step" % (1e3 * dt32 / a.steps, 1e3 * dt64 / a.steps))
+    # accuracy of one mixed matvec against the FP64 one on the same input
+    y64 = plan.matvec(x, prec=F.FMB_FP64)
+    y32 = plan.matvec(x, prec=F.FMB_MIXED)
+    err_mv = float(torch.linalg.norm(y32 - y64) / torch.linalg.norm(y64))
+    del y64, y32
 
     # dominant kernel: the M2L launch of the level with the largest time
     lbig = max(m2l32, key=lambda l: m2l32[l])
@@ -173,10 +194,19 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        val, cores, csetup, cdt = cpu_baseline(V, Fc, k, w["depth"], 30.0)
+        st = w["cpu_stride"]
+        val, cores, csetup, csample, cest = cpu_baseline(V, Fc, k, w["depth"], st)
+        what = ("one FP64 FMM matvec (one right-hand side) of the same %s mesh, tree and "
+                "orders by the C/OpenMP oracle" % a.workload)
+        if st > 1:
+            what += (", timed on a sample: P2M/M2M whole, M2L/L2L/L2P/near on the first 1/%d "
+                     "of the leaf boxes and their ancestors, each level scaled to all its "
+                     "boxes (sample %.1fs, estimated full matvec %.1fs" % (st, csample, cest))
+        else:
+            what += " (matvec %.1fs" % cest
+        what += "; oracle setup %.1fs excluded)" % csetup
         cpu = {"value": val, "unit": "matvecs/s", "cores": cores, "kind": "port",
-               "sample": "one FP64 FMM matvec of the same %s mesh and tree by the C/OpenMP "
-                         "oracle (setup %.1fs excluded, matvec %.1fs)" % (a.workload, csetup, cdt)}
+               "sample": what}
 
     out = {
         "metric": "FMM matvecs/s + time-to-1e-6 GMRES solve, mixed vs FP64, 1 GPU",
@@ -197,10 +227,12 @@ def main():
                    "parallelism": "1 GPU"},
         "fp64_matvecs_per_s": mv64,
         "speedup_mixed_vs_fp64_matvec": mv32 / mv64,
+        "matvec_rel_err_mixed_vs_fp64": err_mv,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK["fp32"],
-                     "unit": "TFLOP/s", "frac": ach32 / PEAK["fp32"], "traffic": None,
-                     "kernel": "k_m2l level %d (FP32)" % lbig,
+                     "unit": "TFLOP/s", "frac": ach32 / PEAK["fp32"],
+                     "traffic": traffic_of(a.workload, lbig),
+                     "kernel": "M2L level %d, FP32 matrix-core kernel" % lbig,
                      "fp64_achieved": ach64, "fp64_frac": ach64 / PEAK["fp64"],
                      "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
                      "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},

@@ -63,6 +63,7 @@ typedef struct fmb_opset {
   void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
   void *d_gws;       /* global workspace of the GWS-mode translations       */
   void *d_Hpad;      /* zero-padded rotation blocks [nbeta][hpad_stride]    */
+  void *d_phase[3];  /* e^{i m phi}, m < Pmax: M2L offsets, M2M, L2L octants */
   size_t hpad_stride;
   int gws_grid;      /* persistent workgroups sharing d_gws                  */
 } fmb_opset;

@@ -79,6 +79,8 @@ typedef struct fmbh_rcr_job {
   const void *chunks;
   int nchunks, chunk_pairs;
   const void *info;
+  const void *phase; /* per bucket: e^{i m phi}, m < pstride */
+  int pstride;
   const int *tgt, *src;
   const void *H;
   size_t hstride;

@@ -80,6 +80,8 @@ struct RcrArgs {
   const int4 *chunks;  // {bucket, first pair, end pair, 0}
   int nchunks, nb;     // chunks, batches per chunk (grid y / item stride)
   const OffInfo *info; // per bucket: beta index, coaxial matrix index, phi
+  const void *phase;   // per bucket: e^{i m phi}, m < pstride
+  int pstride;
   const int *ptgt, *psrc;
   const void *H;       // padded rotation blocks, per beta
   size_t hstride;
@@ -157,53 +159,58 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
   }
 }
 
-// Row tiles [t0, t0 + NTL) of coaxial block m: rows (n', m), n' = |m| ..
-// pout-1, from rows (n, m), n = |m| .. pin-1.  Ab: the block (padded,
-// leading dimension pad16(so)).
-template <typename T, int NT, int NTL>
+// Row tiles [t0, t0 + NTL) of the coaxial blocks +am and (SG = 2) -am, which
+// share the matrix C^{am}: rows (n', +-am), n' = am .. pout-1, from rows
+// (n, +-am), n = am .. pin-1.  Ab: the block (padded, leading dimension
+// pad16(so)).
+template <typename T, int NT, int NTL, int SG>
 __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
                                           const cplx<T> *__restrict__ Ab,
-                                          int m, int pin, int pout, int t0) {
+                                          int am, int pin, int pout, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
-  constexpr int RS = Row<NT>::stride, KU = Ahead<T, NTL>::ku;
+  constexpr int RS = Row<NT>::stride, NB = SG * NT;
+  constexpr int KU = (sizeof(T) == 4 && NTL * SG <= 3) ? 4 : 2;
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   const bool odd = li & 1;
-  const int am = m < 0 ? -m : m, so = pout - am, si = pin - am;
+  const int so = pout - am, si = pin - am;
   const int ld = pad16(so), nks = (si + 3) >> 2, ng = (nks + KU - 1) / KU;
   const cplx<T> *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
-  acc_t acc[NTL][NT];
+  acc_t acc[NTL][NB];
 #pragma unroll
   for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
-  cplx<T> a0[KU][NTL], z0[KU][NT], a1[KU][NTL], z1[KU][NT];
-  auto fetch = [&](int g, cplx<T>(&fa)[KU][NTL], cplx<T>(&fz)[KU][NT]) {
+    for (int t = 0; t < NB; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
+  cplx<T> a0[KU][NTL], z0[KU][NB], a1[KU][NTL], z1[KU][NB];
+  auto fetch = [&](int g, cplx<T>(&fa)[KU][NTL], cplx<T>(&fz)[KU][NB]) {
 #pragma unroll
     for (int u = 0; u < KU; u++) {
       const int ks = min(g * KU + u, nks - 1);
-      const int nn = am + min(4 * ks + lk, si - 1), row = nn * nn + nn + m;
-      const T *xr = Xi + row * RS + (li & ~1);
+      const int nn = am + min(4 * ks + lk, si - 1), rb = nn * nn + nn;
 #pragma unroll
-      for (int t = 0; t < NT; t++) fz[u][t] = *(const cplx<T> *)(xr + 16 * t);
+      for (int sg = 0; sg < SG; sg++) {
+        const T *xr = Xi + (rb + (sg ? -am : am)) * RS + (li & ~1);
+#pragma unroll
+        for (int t = 0; t < NT; t++) fz[u][sg * NT + t] = *(const cplx<T> *)(xr + 16 * t);
+      }
       const cplx<T> *ak = ap + (unsigned)(4 * ks * ld);
 #pragma unroll
       for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
     }
   };
-  auto mma = [&](int g, const cplx<T>(&fa)[KU][NTL], const cplx<T>(&fz)[KU][NT]) {
+  auto mma = [&](int g, const cplx<T>(&fa)[KU][NTL], const cplx<T>(&fz)[KU][NB]) {
 #pragma unroll
     for (int u = 0; u < KU; u++) {
       if (g * KU + u < nks) {
-        T b[NT], bq[NT];
+        T b[NB], bq[NB];
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
+        for (int t = 0; t < NB; t++) {
           b[t] = odd ? fz[u][t].im : fz[u][t].re;
           bq[t] = odd ? fz[u][t].re : -fz[u][t].im;
         }
 #pragma unroll
         for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-          for (int t = 0; t < NT; t++) {
+          for (int t = 0; t < NB; t++) {
             acc[mt][t] = Mfma<T>::mma(fa[u][mt].re, b[t], acc[mt][t]);
             acc[mt][t] = Mfma<T>::mma(fa[u][mt].im, bq[t], acc[mt][t]);
           }
@@ -225,9 +232,12 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
     for (int e = 0; e < 4; e++) {
       const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
       if (i < so) {
-        const int nn = am + i, row = nn * nn + nn + m;
+        const int nn = am + i, rb = nn * nn + nn;
 #pragma unroll
-        for (int t = 0; t < NT; t++) Xo[row * RS + 16 * t + li] = acc[mt][t][e];
+        for (int sg = 0; sg < SG; sg++)
+#pragma unroll
+          for (int t = 0; t < NT; t++)
+            Xo[(rb + (sg ? -am : am)) * RS + 16 * t + li] = acc[mt][sg * NT + t][e];
       }
     }
   }
@@ -246,17 +256,17 @@ __device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo,
   rot_unit<T, NT, MT>(Xi, Xo, Ab, s, base, t0);
 }
 
-template <typename T, int NT, int MT>
+template <typename T, int NT, int MT, int SG>
 __device__ __forceinline__ void coax_any(int ntl, const T *Xi, T *Xo,
-                                         const cplx<T> *Ab, int m, int pin,
+                                         const cplx<T> *Ab, int am, int pin,
                                          int pout, int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      coax_any<T, NT, MT - 1>(ntl, Xi, Xo, Ab, m, pin, pout, t0);
+      coax_any<T, NT, MT - 1, SG>(ntl, Xi, Xo, Ab, am, pin, pout, t0);
       return;
     }
   }
-  coax_unit<T, NT, MT>(Xi, Xo, Ab, m, pin, pout, t0);
+  coax_unit<T, NT, MT, SG>(Xi, Xo, Ab, am, pin, pout, t0);
 }
 
 // Rotation stage over this wave's units {n*16 + g, operator offset}: row
@@ -273,17 +283,20 @@ __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
   }
 }
 
-// Coaxial stage over this wave's units {(m + P)*16 + g, operator offset}.
+// Coaxial stage over this wave's units {am*16 + g, operator offset}: the
+// blocks +am and -am together.
 template <typename T, int NT, int MT>
 __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
-                                           const int *units, int count, int P,
-                                           int pin, int pout) {
+                                           const int *units, int count, int pin,
+                                           int pout) {
   for (int j = 0; j < count; j++) {
     const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
-    const int m = (code >> 4) - P, t0 = (code & 15) * MT;
-    const int so = pout - (m < 0 ? -m : m);
-    const int ntl = min(MT, ((so + 15) >> 4) - t0);
-    coax_any<T, NT, MT>(ntl, Xi, Xo, C + aoff, m, pin, pout, t0);
+    const int am = code >> 4, t0 = (code & 15) * MT;
+    const int ntl = min(MT, ((pout - am + 15) >> 4) - t0);
+    if (am == 0)
+      coax_any<T, NT, MT, 1>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
+    else
+      coax_any<T, NT, MT, 2>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
   }
 }
 
@@ -294,7 +307,7 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
                                           const int *sched) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
   const int tid = threadIdx.x, wave = uni(tid >> 6);
-  const int pin = a.pin, pout = a.pout, P = a.P;
+  const int pin = a.pin, pout = a.pout;
   const int Pm = pin > pout ? pin : pout;
   const int Pi2 = pin * pin, Po2 = pout * pout, Pm2 = Pm * Pm;
   const OffInfo oi = a.info[ch.x];
@@ -302,11 +315,8 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   const cplx<T> *C = (const cplx<T> *)a.C + a.cstride * oi.dist;
   const int U = (ch.z - ch.y) * a.nrhs;
   __syncthreads();
-  for (int m = tid; m < Pm; m += kThreads) {
-    double sn, cs;
-    sincos(m * oi.phi, &sn, &cs);
-    ph[m] = cmk<T>((T)cs, (T)sn);
-  }
+  const cplx<T> *phb = (const cplx<T> *)a.phase + (size_t)a.pstride * ch.x;
+  for (int m = tid; m < Pm; m += kThreads) ph[m] = phb[m];
   // vector v of the batch: (pair, right-hand side) = divmod(u0 + v, nrhs);
   // its source and target base pointers are wave-uniform
   const cplx<T> *vin[V];
@@ -344,7 +354,7 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   const int *u3 = sched + 2 * uni(o3[wave]), w3 = uni(o3[wave + 1] - o3[wave]);
   rot_stage<T, NT, MT>(X0, X1, H, u1, w1);
   __syncthreads();
-  coax_stage<T, NT, MT>(X1, X0, C, uc, wc, P, pin, pout);
+  coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
   __syncthreads();
   rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
   __syncthreads();
@@ -361,8 +371,11 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
 }
 
 // LDS mode: grid (nchunks, nb), one batch per workgroup.
+// FP32 batches ask for 3 waves per SIMD (the LDS of three workgroups fits a
+// CU); FP64 keeps its accumulators unspilled instead.
 template <typename T, int NT, int MT>
-__global__ void __launch_bounds__(kThreads) k_rcr_lds(RcrArgs a) {
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) k_rcr_lds(RcrArgs a) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
   extern __shared__ __align__(16) unsigned char smem[];
   const int Pm = a.pin > a.pout ? a.pin : a.pout;
@@ -462,6 +475,8 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   a.nchunks = j->nchunks;
   a.nb = (j->chunk_pairs * nrhs + V - 1) / V;
   a.info = (const OffInfo *)j->info;
+  a.phase = j->phase;
+  a.pstride = j->pstride;
   a.ptgt = j->tgt;
   a.psrc = j->src;
   a.H = j->H;

@@ -109,6 +109,7 @@ static void opset_free(fmb_opset *o) {
   fmbh_free(o->d_ys);
   fmbh_free(o->d_gws);
   fmbh_free(o->d_Hpad);
+  for (int i = 0; i < 3; i++) fmbh_free(o->d_phase[i]);
   memset(o, 0, sizeof(*o));
 }
 

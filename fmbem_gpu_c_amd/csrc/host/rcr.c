@@ -13,6 +13,7 @@
  * translation are split into units (whole blocks when the batch sits in LDS
  * and is updated in place; row groups otherwise) and assigned to the
  * workgroup's waves by longest-processing-time first. */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -107,16 +108,17 @@ static int rot_units(int np, int split, int mt, const int *hoff, unit_t *u) {
   return k;
 }
 
-/* coaxial blocks |m| < min(pin, pout): code (m + P)*16 + g */
-static int coax_units(int pin, int pout, int P, int split, int mt, const int *coff,
+/* coaxial blocks +-am, am < min(pin, pout), processed together (they share
+ * the matrix C^{am}): code am*16 + g */
+static int coax_units(int pin, int pout, int split, int mt, const int *coff,
                       unit_t *u) {
   int k = 0, mn = pin < pout ? pin : pout;
-  for (int m = -(mn - 1); m <= mn - 1; m++) {
-    int am = m < 0 ? -m : m, so = pout - am, si = pin - am;
+  for (int am = 0; am < mn; am++) {
+    int so = pout - am, si = pin - am, sg = am ? 2 : 1;
     int nt = tiles(so), ng = split ? (nt + mt - 1) / mt : 1;
     for (int g = 0; g < ng; g++) {
-      u[k].code = (m + P) * 16 + g;
-      u[k].cost = 2 * group_tiles(nt, g, mt, split) * ksteps(si);
+      u[k].code = am * 16 + g;
+      u[k].cost = 2 * sg * group_tiles(nt, g, mt, split) * ksteps(si);
       u[k].aoff = coff[am];
       k++;
     }
@@ -134,7 +136,7 @@ static int build_sched(int pin, int pout, int P, int mode, int mt, const int *ho
   int split = mode == FMBH_RCR_GWS, rc;
   unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + 2 * P + pout));
   int n1 = rot_units(pin, split, mt, hoff, u);
-  int nc = coax_units(pin, pout, P, split, mt, coff, u + n1);
+  int nc = coax_units(pin, pout, split, mt, coff, u + n1);
   int n3 = rot_units(pout, split, mt, hoff, u + n1 + nc);
   int tot = RCR_SCHED_HDR + 2 * (n1 + nc + n3);
   int *h = (int *)calloc(tot, sizeof(int));
@@ -256,6 +258,34 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     if ((rc = fmbh_h2d(pl->d_info_oct[kind - 1], info, sizeof(info), stream))) return rc;
   }
   if (pl->prm.rcr_path != FMB_RCR_VALU) {
+    /* e^{i m phi} of every bucket, m < Pmax, per opset */
+    int nph[3] = {FMB_NOFF, 8, 8};
+    for (int q = 0; q < 3 && !rc; q++) {
+      double *ph = (double *)malloc(sizeof(double) * 2 * nph[q] * pl->Pmax);
+      for (int b = 0; b < nph[q]; b++) {
+        double phi = q == 0 ? pl->off_phi[b] : (q == 1 ? pl->m2m_phi[b] : pl->l2l_phi[b]);
+        for (int m = 0; m < pl->Pmax; m++) {
+          ph[2 * ((size_t)b * pl->Pmax + m)] = cos(m * phi);
+          ph[2 * ((size_t)b * pl->Pmax + m) + 1] = sin(m * phi);
+        }
+      }
+      for (int op = 0; op < pl->nops && !rc; op++) {
+        fmb_opset *o = &pl->ops[op];
+        size_t n = (size_t)2 * nph[q] * pl->Pmax;
+        rc = fmbh_malloc(&o->d_phase[q], o->csz / 2 * n);
+        if (rc) break;
+        if (o->prec == FMB_FP64) {
+          rc = fmbh_h2d(o->d_phase[q], ph, sizeof(double) * n, stream);
+        } else {
+          float *f = (float *)malloc(sizeof(float) * n);
+          for (size_t i = 0; i < n; i++) f[i] = (float)ph[i];
+          rc = fmbh_h2d(o->d_phase[q], f, sizeof(float) * n, stream);
+          free(f);
+        }
+      }
+      free(ph);
+    }
+    if (rc) return rc;
     /* padded rotation blocks of every polar angle, per opset */
     int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1)), *d_hoff = NULL;
     rot_offsets(pl->Pmax, hoff);
@@ -359,6 +389,8 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.sched_len = tb->sched_len[op];
   j.gws = o->d_gws;
   j.gws_grid = o->gws_grid;
+  j.phase = o->d_phase[kind];
+  j.pstride = pl->Pmax;
   if (kind == 0) {
     j.info = pl->d_off_info;
     j.tgt = t->d_pair_tgt[l];

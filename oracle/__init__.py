@@ -75,6 +75,7 @@ _SIGS = {
     "fo_bem_fmm_setup": (_ci, [_vp, _ci, _ci, _vp]),
     "fo_bem_fmm_orders": (_ci, [_vp, _ip]),
     "fo_bem_fmm_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
+    "fo_bem_fmm_time_sample": (ctypes.c_double, [_vp, _ci, _dp]),
     "fo_bem_near_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_gmres": (_ci, [_vp, _ci, _ci, _cd, _ci, _dp, _dp, _dp, _dp, ctypes.POINTER(_ci),
                        ctypes.POINTER(_cd)]),
@@ -355,6 +356,13 @@ class Bem:
 
     def near_matvec(self, x):
         return self._mv(lib().fo_bem_near_matvec, x)
+
+    def fmm_time_sample(self, stride):
+        """Estimated seconds of one FMM matvec (see fo_bem_fmm_time_sample)
+        and the measured phase seconds of the sample."""
+        ph = np.zeros(4)
+        est = lib().fo_bem_fmm_time_sample(self.h, int(stride), ph)
+        return est, ph[1:]
 
     def dense_row(self, i):
         re, im = np.zeros(self.n), np.zeros(self.n)
