@@ -47,6 +47,7 @@ class PlanParams(ctypes.Structure):
         ("nedge", ctypes.c_int),
         ("nrhs_max", ctypes.c_int),
         ("rcr_path", ctypes.c_int),
+        ("k_max", ctypes.c_double),
     ]
 
 
@@ -101,6 +102,10 @@ _SIGS = {
     "fmb_matvec": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "fmb_matvec_parts": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
+    "fmb_field_eval": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "fmb_plan_retune": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp]),
+    "fmb_plan_wavenumber": (ctypes.c_int, [_vp, _vp]),
+    "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.POINTER(GmresInfo), _vp]),
     "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -255,7 +260,10 @@ def plan_params(k, **kw):
     p = PlanParams()
     lib().fmb_plan_params_default(ctypes.byref(p))
     p.k = k
+    names = {f[0] for f in PlanParams._fields_}
     for key, v in kw.items():
+        if key not in names:
+            raise FmbError(-1, "unknown plan parameter %r" % key)
         if key == "p_levels":
             for i, q in enumerate(v):
                 p.p_levels[i] = int(q)
@@ -274,6 +282,34 @@ class Plan:
         rb = _dev(robin) if robin is not None else None
         _check(lib().fmb_plan_create(tree.h, ctypes.byref(self.params), rb, _stream(),
                                      ctypes.byref(self.h)))
+
+    def retune(self, k, robin=None):
+        """Re-targets the plan at wavenumber k (fmb_plan_retune); robin as
+        for the constructor, at the new frequency."""
+        rb = _dev(robin) if robin is not None else None
+        _check(lib().fmb_plan_retune(self.h, float(k), rb, _stream()))
+
+    @property
+    def k(self):
+        v = ctypes.c_double()
+        _check(lib().fmb_plan_wavenumber(self.h, ctypes.byref(v)))
+        return v.value
+
+    def sweep_guess(self, k_from, x_from, dirs=None):
+        """Initial guess at the plan's wavenumber from the solution x_from at
+        k_from, carrying each right-hand side's incident phase."""
+        import torch
+        nrhs = 1 if x_from.dim() == 1 else x_from.shape[0]
+        d = None
+        if dirs is not None:
+            d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+            if d.shape[0] != nrhs:
+                raise FmbError(-1, "one incident direction per right-hand side")
+        x0 = torch.empty_like(x_from)
+        _check(lib().fmb_sweep_guess(self.h, float(k_from), _dev(x_from),
+                                     ctypes.c_void_p(d.ctypes.data) if d is not None else None,
+                                     nrhs, _dev(x0), _stream()))
+        return x0
 
     def orders(self):
         p = np.zeros(16, dtype=np.int32)
@@ -308,6 +344,26 @@ class Plan:
         _check(lib().fmb_rhs_plane_wave(self.h, _ptr(dirs, ctypes.c_double), dirs.shape[0],
                                         _dev(b), _stream()))
         return b
+
+    def field(self, x, pts, dirs=None):
+        """Pressure at field points pts (M x 3) from the surface solution x
+        (complex128 CUDA tensor, [nrhs][N] or [N]); dirs: incident unit
+        plane waves (nrhs x 3) added to the scattered field, or None.
+        Returns a [nrhs][M] complex128 CUDA tensor."""
+        import torch
+        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        d = None
+        if dirs is not None:
+            d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+            if d.shape[0] != nrhs:
+                raise FmbError(-1, "one incident direction per right-hand side")
+        out = torch.empty((nrhs, pts.shape[0]), dtype=torch.complex128, device="cuda")
+        _check(lib().fmb_field_eval(self.h, _dev(x), nrhs, ctypes.c_void_p(pts.ctypes.data),
+                                    pts.shape[0],
+                                    ctypes.c_void_p(d.ctypes.data) if d is not None else None,
+                                    _dev(out), _stream()))
+        return out
 
     def gmres(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
         import torch

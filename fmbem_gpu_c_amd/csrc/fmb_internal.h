@@ -86,7 +86,7 @@ struct fmb_plan {
   fmb_plan_params prm;
   double k, alpha_re, alpha_im;
   int p[FMB_MAXLEV];
-  int Pmax;
+  int Pmax;  /* order of the rotation tables: >= every level's order        */
   double *h_scale[FMB_MAXLEV]; /* w_n per level                              */
   double *d_scale[FMB_MAXLEV];
   int nbeta;
@@ -120,10 +120,13 @@ struct fmb_plan {
   int work_prec;        /* precision of the Arnoldi matvecs in fmb_gmres   */
   int nrhs_max;
   double *d_work64;     /* [nrhs][N] FP64 complex scratch                  */
+  fmb_c64 *d_robin;     /* Robin coefficients in Morton order, or NULL     */
   int timing;
   double t_phase[7];
   double t_m2l[FMB_MAXLEV];
-  size_t bytes;
+  size_t bytes;         /* device bytes of the rotation + frequency tiers  */
+  size_t bytes_rot, bytes_freq;
+  int ready;            /* frequency tier built (cleared by a failed retune) */
   void *stream;
 };
 

@@ -131,6 +131,25 @@ __global__ void k_rhs_plane(const double *__restrict__ geo,
   b[t] = make_double2(c * fr - s * fi, c * fi + s * fr);
 }
 
+// x0 = x e^{i dk d_r . c_e}, caller order; in place when x0 == x
+__global__ void k_sweep_guess(const double *__restrict__ geo,
+                              const int *__restrict__ iperm,
+                              const double *__restrict__ dirs, int nrhs, int N,
+                              double dk, const double2 *x, double2 *x0) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * nrhs) return;
+  int r = (int)(t / N), e = (int)(t % N);
+  double2 v = x[t];
+  if (dirs) {
+    const double *g = geo + 8 * (size_t)iperm[e];
+    const double *d = dirs + 3 * r;
+    double s, c;
+    sincos(dk * (d[0] * g[0] + d[1] * g[1] + d[2] * g[2]), &s, &c);
+    v = make_double2(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+  x0[t] = v;
+}
+
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -214,4 +233,23 @@ extern "C" int fmbh_rhs_plane(const fmb_plan *pl, const double *h_dirs,
   FMB_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   fmbh_free(d_dirs);
   return FMB_OK;
+}
+
+extern "C" int fmbh_sweep_guess(const fmb_plan *pl, double dk, const fmb_c64 *d_x,
+                                const double *h_dirs, int nrhs, fmb_c64 *d_x0,
+                                void *stream) {
+  const fmb_tree *t = pl->tree;
+  double *d_dirs = NULL;
+  int rc;
+  if (h_dirs) {
+    if ((rc = fmbh_malloc((void **)&d_dirs, sizeof(double) * 3 * nrhs))) return rc;
+    if ((rc = fmbh_h2d(d_dirs, h_dirs, sizeof(double) * 3 * nrhs, stream))) return rc;
+  }
+  size_t n = (size_t)t->nelem * nrhs;
+  hipLaunchKernelGGL(k_sweep_guess, dim3(nblk(n)), dim3(256), 0,
+                     (hipStream_t)stream, t->d_geo, t->d_iperm, d_dirs, nrhs,
+                     t->nelem, dk, (const double2 *)d_x, (double2 *)d_x0);
+  FMB_LAUNCH_CHECK();
+  FMB_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return fmbh_free(d_dirs);
 }

@@ -114,6 +114,11 @@ int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
 int fmbh_mmajor_table(int p, int **d_tab, void *stream);
 
 /* ---- FP64 vector kernels for GMRES (blas.hip) */
+/* pressure at field points (field.hip): d_pts npts*3, d_dirs nrhs*3 or NULL */
+int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *d_pts,
+               int npts, const double *d_dirs, fmb_c64 *out, void *stream);
+int fmbh_sweep_guess(const fmb_plan *pl, double dk, const fmb_c64 *d_x,
+                     const double *h_dirs, int nrhs, fmb_c64 *d_x0, void *stream);
 int fmbh_rhs_plane(const fmb_plan *pl, const double *h_dirs, int nrhs,
                    fmb_c64 *d_b, void *stream);
 /* h[j*nrhs + r] = <V_j^r, w^r> for j < nv ; V: [nv][nrhs][N] */

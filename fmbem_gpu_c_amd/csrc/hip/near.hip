@@ -87,45 +87,6 @@ __device__ void self_terms(double k, const double *tri, const double *c,
   T = make_double2(-f * (sTr + sinv), -f * (sTi - 2.0 * M_PI * k));
 }
 
-// accumulate S, dG/dny, dG/dnx, d2G over a rule
-__device__ void panel_sum(double k, const double *x, const double *nx,
-                          const double *tri, const double *ny, double area,
-                          const double *quad, int nq, double2 out[4]) {
-  for (int i = 0; i < 4; i++) out[i] = make_double2(0, 0);
-  for (int q = 0; q < nq; q++) {
-    const double *qq = quad + 4 * q;
-    double y[3], d[3];
-    for (int a = 0; a < 3; a++) {
-      y[a] = qq[0] * tri[a] + qq[1] * tri[3 + a] + qq[2] * tri[6 + a];
-      d[a] = x[a] - y[a];
-    }
-    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double s, c;
-    sincos(k * r, &s, &c);
-    double w = qq[3] * area;
-    double g0 = 1.0 / (4.0 * M_PI * r);
-    double gr = c * g0, gi = s * g0;
-    double rnx = (d[0] * nx[0] + d[1] * nx[1] + d[2] * nx[2]) / r;
-    double rny = -(d[0] * ny[0] + d[1] * ny[1] + d[2] * ny[2]) / r;
-    // gp = g (-1/r + i k)
-    double gpr = -gr / r - k * gi, gpi = -gi / r + k * gr;
-    double nn = nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2];
-    // d2G = e/(4 pi r^3) [ (3 - k^2 r^2 - 3ikr) rnx rny + (1 - ikr) nn ]
-    double ar = (3.0 - k * k * r * r) * rnx * rny + nn;
-    double ai = -3.0 * k * r * rnx * rny - k * r * nn;
-    double h0 = g0 / (r * r);
-    double hr = h0 * (c * ar - s * ai), hi = h0 * (c * ai + s * ar);
-    out[0].x += w * gr;
-    out[0].y += w * gi;
-    out[1].x += w * gpr * rny;
-    out[1].y += w * gpi * rny;
-    out[2].x += w * gpr * rnx;
-    out[2].y += w * gpi * rnx;
-    out[3].x += w * hr;
-    out[3].y += w * hi;
-  }
-}
-
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

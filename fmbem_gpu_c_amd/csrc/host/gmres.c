@@ -33,7 +33,7 @@ static fmb_c64 fromz(zc z) {
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream) {
-  if (!pl || !b || !x || nrhs < 1 || nrhs > pl->nrhs_max || restart < 1 ||
+  if (!pl || !pl->ready || !b || !x || nrhs < 1 || nrhs > pl->nrhs_max || restart < 1 ||
       !(tol > 0) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
   double t0 = now_s();

@@ -5,7 +5,10 @@
 void fmbx_gauss_legendre(int n, double *x, double *w);
 int fmbx_tri_rule(int rule, int order, double **out);
 void fmbx_abs_hn(int nmax, double x, double *out);
-/* matrix-core translations (rcr.c) */
+/* matrix-core translations (rcr.c): phase tables and padded rotation blocks
+ * of every opset (rotation tier), then schedules, padded coaxial blocks and
+ * workspaces (frequency tier) */
+int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream);
 int fmbx_rcr_setup(fmb_plan *pl, void *stream);
 void fmbx_rcr_free(fmb_plan *pl);
 int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,

@@ -1,6 +1,21 @@
 /* plan.c -- per-frequency plan (the operator half of genOctree,
  * translation.cu:3078-3348, plus bases and near field) and the FMM pass
- * (OprL, translation.cu:3412-3769, completed with L2P and near field). */
+ * (OprL, translation.cu:3412-3769, completed with L2P and near field).
+ *
+ * A plan has three tiers of state:
+ *   geometry  -- translation directions and their polar-angle classes, octant
+ *                geometry, M2L work chunks, quadrature rules, near-field
+ *                layout: built once per tree;
+ *   rotation  -- real rotation blocks H_n(beta) of every polar angle at order
+ *                Pmax, padded copies and azimuth phase tables: independent of
+ *                the wavenumber, rebuilt only when an order grows past Pmax
+ *                (genOctree builds rotMat1/rotMat2 once for the largest p,
+ *                translation.cu:3160-3174);
+ *   frequency -- orders, scale factors, coaxial matrices, bases, near-field
+ *                blocks, Robin coefficients and translation schedules:
+ *                rebuilt by fmb_plan_retune (genOctree regenerates only the
+ *                coaxial matrices per wavenumber, translation.cu:3176-3186,
+ *                3294-3302). */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -54,15 +69,23 @@ static int order_for(const fmb_plan_params *prm, double k, double side, int l) {
   return p;
 }
 
-static int opset_alloc(fmb_plan *pl, fmb_opset *o, int prec) {
+/* orders of levels 2..lmax at wavenumber k and their maximum */
+static int level_orders(const fmb_plan *pl, double k, int *p, int *pmax) {
+  const fmb_tree *t = pl->tree;
+  *pmax = 0;
+  for (int l = 2; l <= t->lmax; l++) {
+    p[l] = order_for(&pl->prm, k, t->D * ldexp(1.0, -l), l);
+    if (p[l] < 1 || p[l] > 200)
+      return fmb_set_error(FMB_ERR_RANGE, "order %d at level %d (k = %g)", p[l], l, k);
+    if (p[l] > *pmax) *pmax = p[l];
+  }
+  return FMB_OK;
+}
+
+/* wavenumber-dependent arrays of one opset */
+static int opset_alloc(fmb_plan *pl, fmb_opset *o) {
   const fmb_tree *t = pl->tree;
   int rc, L = t->lmax, N = t->nelem, nr = pl->nrhs_max;
-  o->prec = prec;
-  o->csz = prec == FMB_FP64 ? 16 : 8;
-  size_t rsz = o->csz / 2;
-  size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
-  if ((rc = fmbh_malloc(&o->d_H, rsz * hs))) return rc;
-  pl->bytes += rsz * hs;
   for (int l = 2; l <= L; l++) {
     size_t cs = (size_t)fmb_coax_size(pl->p[l]) * FMB_NDIST * o->csz;
     if ((rc = fmbh_malloc(&o->d_Csr[l], cs))) return rc;
@@ -93,14 +116,14 @@ static int opset_alloc(fmb_plan *pl, fmb_opset *o, int prec) {
   return FMB_OK;
 }
 
-static void opset_free(fmb_opset *o) {
-  fmbh_free(o->d_H);
+static void opset_free_freq(fmb_opset *o) {
   for (int l = 0; l < FMB_MAXLEV; l++) {
     fmbh_free(o->d_Csr[l]);
     fmbh_free(o->d_Cm2m[l]);
     fmbh_free(o->d_Cl2l[l]);
     fmbh_free(o->d_M[l]);
     fmbh_free(o->d_L[l]);
+    o->d_Csr[l] = o->d_Cm2m[l] = o->d_Cl2l[l] = o->d_M[l] = o->d_L[l] = NULL;
   }
   fmbh_free(o->d_P2M);
   fmbh_free(o->d_L2P);
@@ -108,21 +131,51 @@ static void opset_free(fmb_opset *o) {
   fmbh_free(o->d_xs);
   fmbh_free(o->d_ys);
   fmbh_free(o->d_gws);
+  o->d_P2M = o->d_L2P = o->d_near = o->d_xs = o->d_ys = o->d_gws = NULL;
+  o->gws_grid = 0;
+}
+
+static void opset_free_rot(fmb_opset *o) {
+  fmbh_free(o->d_H);
   fmbh_free(o->d_Hpad);
-  for (int i = 0; i < 3; i++) fmbh_free(o->d_phase[i]);
-  memset(o, 0, sizeof(*o));
+  o->d_H = o->d_Hpad = NULL;
+  for (int i = 0; i < 3; i++) {
+    fmbh_free(o->d_phase[i]);
+    o->d_phase[i] = NULL;
+  }
+  o->hpad_stride = 0;
+}
+
+static void plan_freq_free(fmb_plan *pl) {
+  pl->ready = 0;
+  fmbx_rcr_free(pl);
+  for (int i = 0; i < 2; i++) opset_free_freq(&pl->ops[i]);
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    free(pl->h_scale[l]);
+    fmbh_free(pl->d_scale[l]);
+    fmbh_free(pl->d_tab_mmajor[l]);
+    pl->h_scale[l] = NULL;
+    pl->d_scale[l] = NULL;
+    pl->d_tab_mmajor[l] = NULL;
+  }
+  fmbh_free(pl->d_robin);
+  pl->d_robin = NULL;
+  pl->bytes -= pl->bytes_freq;
+  pl->bytes_freq = 0;
+}
+
+static void plan_rot_free(fmb_plan *pl) {
+  for (int i = 0; i < 2; i++) opset_free_rot(&pl->ops[i]);
+  pl->bytes -= pl->bytes_rot;
+  pl->bytes_rot = 0;
+  pl->Pmax = 0;
 }
 
 void fmb_plan_destroy(fmb_plan *pl) {
   if (!pl) return;
-  fmbx_rcr_free(pl);
-  for (int i = 0; i < 2; i++) opset_free(&pl->ops[i]);
-  for (int l = 0; l < FMB_MAXLEV; l++) {
-    free(pl->h_scale[l]);
-    fmbh_free(pl->d_scale[l]);
-    fmbh_free(pl->d_m2l_chunks[l]);
-    fmbh_free(pl->d_tab_mmajor[l]);
-  }
+  plan_freq_free(pl);
+  plan_rot_free(pl);
+  for (int l = 0; l < FMB_MAXLEV; l++) fmbh_free(pl->d_m2l_chunks[l]);
   fmbh_free(pl->d_m2m_beta);
   fmbh_free(pl->d_l2l_beta);
   fmbh_free(pl->d_m2m_phi);
@@ -142,95 +195,12 @@ typedef struct {
   double phi;
 } off_info_t;
 
-/* operators of one precision from the FP64 originals */
-static int build_opset(fmb_plan *pl, fmb_opset *o, const double *d_H64,
-                       fmb_c64 *const *d_sr64, fmb_c64 *const *d_rr64,
-                       const fmb_c64 *d_robin) {
+/* translation directions, octant geometry, M2L work chunks, quadrature rules
+ * and near-field layout: everything fixed by the tree and the rules */
+static int plan_geometry(fmb_plan *pl, void *stream) {
   const fmb_tree *t = pl->tree;
+  const fmb_plan_params *prm = &pl->prm;
   int rc, L = t->lmax;
-  void *st = pl->stream;
-  size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
-  if ((rc = fmbh_convert_real(o->prec, d_H64, hs, o->d_H, st))) return rc;
-  for (int l = 2; l <= L; l++) {
-    int p = pl->p[l];
-    size_t cs = (size_t)fmb_coax_size(p);
-    for (int d = 0; d < FMB_NDIST; d++) {
-      /* M2L: L~ = w L, M = w M~  ->  diag(w) (S|R) diag(w) */
-      if ((rc = fmbh_coax_scale(o->prec, d_sr64[l] + cs * d, p, pl->d_scale[l],
-                                pl->d_scale[l], 0, 0,
-                                (char *)o->d_Csr[l] + cs * d * o->csz, st)))
-        return rc;
-    }
-    if (l > 2) {
-      int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
-      /* M2M child l -> parent l-1: diag(1/w_{l-1}) (R|R) diag(w_l) */
-      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l - 1],
-                                pl->d_scale[l], 1, 0, o->d_Cm2m[l], st)))
-        return rc;
-      /* L2L parent l-1 -> child l: diag(w_l) (R|R) diag(1/w_{l-1}) */
-      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l],
-                                pl->d_scale[l - 1], 0, 1, o->d_Cl2l[l], st)))
-        return rc;
-    }
-  }
-  if ((rc = fmbh_bases(pl, (int)(o - pl->ops), d_robin, st))) return rc;
-  if ((rc = fmbh_near_assemble(pl, (int)(o - pl->ops), d_robin, st))) return rc;
-  return fmbh_sync(st);
-}
-
-int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
-                    const fmb_c64 *robin, void *stream, fmb_plan **out) {
-  if (!t || !prm || !out || !(prm->k > 0) || prm->nrhs_max < 1 ||
-      (prm->precision != FMB_FP64 && prm->precision != FMB_MIXED))
-    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: invalid parameters");
-  if (prm->nedge < 1 || prm->nedge > 256 || prm->near_ratio < 0)
-    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: quadrature parameters");
-  if (prm->rcr_path != FMB_RCR_AUTO && prm->rcr_path != FMB_RCR_VALU)
-    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
-  int rc;
-  fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
-  pl->tree = t;
-  pl->prm = *prm;
-  pl->stream = stream;
-  pl->k = prm->k;
-  if (prm->alpha_re == 0.0 && prm->alpha_im == 0.0) {
-    pl->alpha_re = 0.0;
-    pl->alpha_im = 1.0 / prm->k; /* alpha = i/k */
-  } else {
-    pl->alpha_re = prm->alpha_re;
-    pl->alpha_im = prm->alpha_im;
-  }
-  pl->nrhs_max = prm->nrhs_max;
-  int L = t->lmax;
-  fmb_c64 *d_robin = NULL;
-  double *d_H64 = NULL;
-  fmb_c64 *d_sr64[FMB_MAXLEV] = {0}, *d_rr64[FMB_MAXLEV] = {0};
-  /* orders */
-  pl->Pmax = 0;
-  for (int l = 2; l <= L; l++) {
-    double side = t->D * ldexp(1.0, -l);
-    pl->p[l] = order_for(prm, pl->k, side, l);
-    if (pl->p[l] < 1 || pl->p[l] > 200) {
-      rc = fmb_set_error(FMB_ERR_RANGE, "order %d at level %d", pl->p[l], l);
-      goto fail;
-    }
-    if (pl->p[l] > pl->Pmax) pl->Pmax = pl->p[l];
-  }
-  /* scale factors w_n = min(1, 1/(x |h_n(x)|)), x = k * box radius */
-  for (int l = 2; l <= L; l++) {
-    double x = pl->k * t->D * ldexp(1.0, -l) * sqrt(3.0) / 2.0;
-    int n1 = pl->Pmax + 1;
-    pl->h_scale[l] = (double *)malloc(sizeof(double) * n1);
-    fmbx_abs_hn(n1 - 1, x, pl->h_scale[l]);
-    for (int n = 0; n < n1; n++) {
-      double w = 1.0 / (x * pl->h_scale[l][n]);
-      if (!(w > 1e-280)) w = 1e-280;
-      pl->h_scale[l][n] = w < 1.0 ? w : 1.0;
-    }
-    if ((rc = fmbh_malloc((void **)&pl->d_scale[l], sizeof(double) * n1))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_scale[l], pl->h_scale[l], sizeof(double) * n1, stream)))
-      goto fail;
-  }
   /* directions: M2L offsets and the child/parent diagonals */
   pl->nbeta = 0;
   for (int o = 0; o < FMB_NOFF; o++) {
@@ -259,16 +229,18 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
       info[o].dist = pl->off_dist[o] < 0 ? 0 : pl->off_dist[o];
       info[o].phi = pl->off_phi[o];
     }
-    if ((rc = fmbh_malloc(&pl->d_off_info, sizeof(info)))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_off_info, info, sizeof(info), stream))) goto fail;
-    if ((rc = fmbh_malloc((void **)&pl->d_m2m_beta, sizeof(int) * 8))) goto fail;
-    if ((rc = fmbh_malloc((void **)&pl->d_l2l_beta, sizeof(int) * 8))) goto fail;
-    if ((rc = fmbh_malloc((void **)&pl->d_m2m_phi, sizeof(double) * 8))) goto fail;
-    if ((rc = fmbh_malloc((void **)&pl->d_l2l_phi, sizeof(double) * 8))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_m2m_beta, pl->m2m_beta, sizeof(int) * 8, stream))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_l2l_beta, pl->l2l_beta, sizeof(int) * 8, stream))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_m2m_phi, pl->m2m_phi, sizeof(double) * 8, stream))) goto fail;
-    if ((rc = fmbh_h2d(pl->d_l2l_phi, pl->l2l_phi, sizeof(double) * 8, stream))) goto fail;
+    if ((rc = fmbh_malloc(&pl->d_off_info, sizeof(info)))) return rc;
+    if ((rc = fmbh_h2d(pl->d_off_info, info, sizeof(info), stream))) return rc;
+    if ((rc = fmbh_malloc((void **)&pl->d_m2m_beta, sizeof(int) * 8))) return rc;
+    if ((rc = fmbh_malloc((void **)&pl->d_l2l_beta, sizeof(int) * 8))) return rc;
+    if ((rc = fmbh_malloc((void **)&pl->d_m2m_phi, sizeof(double) * 8))) return rc;
+    if ((rc = fmbh_malloc((void **)&pl->d_l2l_phi, sizeof(double) * 8))) return rc;
+    if ((rc = fmbh_h2d(pl->d_m2m_beta, pl->m2m_beta, sizeof(int) * 8, stream))) return rc;
+    if ((rc = fmbh_h2d(pl->d_l2l_beta, pl->l2l_beta, sizeof(int) * 8, stream))) return rc;
+    if ((rc = fmbh_h2d(pl->d_m2m_phi, pl->m2m_phi, sizeof(double) * 8, stream))) return rc;
+    if ((rc = fmbh_h2d(pl->d_l2l_phi, pl->l2l_phi, sizeof(double) * 8, stream))) return rc;
+    /* the h2d copies read the stack arrays asynchronously */
+    if ((rc = fmbh_sync(stream))) return rc;
   }
   /* M2L work chunks per level: pairs of one offset bucket */
   for (int l = 2; l <= L; l++) {
@@ -290,50 +262,152 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     pl->n_m2l_chunks[l] = n;
     rc = fmbh_malloc(&pl->d_m2l_chunks[l], sizeof(int) * 4 * (n > 0 ? n : 1));
     if (!rc && n) rc = fmbh_h2d(pl->d_m2l_chunks[l], c, sizeof(int) * 4 * n, stream);
+    if (!rc) rc = fmbh_sync(stream);
     free(c);
-    if (rc) goto fail;
-    if ((rc = fmbh_mmajor_table(pl->p[l], &pl->d_tab_mmajor[l], stream))) goto fail;
+    if (rc) return rc;
   }
   /* quadrature tables */
-  {
-    double *q = NULL;
-    int nq = fmbx_tri_rule(prm->quad_rule, prm->quad_order, &q);
-    if (nq < 1) {
-      rc = fmb_set_error(FMB_ERR_ARG, "panel rule %d order %d", prm->quad_rule,
+  double *q = NULL;
+  int nq = fmbx_tri_rule(prm->quad_rule, prm->quad_order, &q);
+  if (nq < 1)
+    return fmb_set_error(FMB_ERR_ARG, "panel rule %d order %d", prm->quad_rule,
                          prm->quad_order);
-      goto fail;
-    }
-    pl->nq = nq;
-    rc = fmbh_malloc((void **)&pl->d_quad, sizeof(double) * 4 * nq);
-    if (!rc) rc = fmbh_h2d(pl->d_quad, q, sizeof(double) * 4 * nq, stream);
-    free(q);
-    if (rc) goto fail;
-    int nqn = fmbx_tri_rule(prm->near_rule, prm->near_order, &q);
-    if (nqn < 1) {
-      rc = fmb_set_error(FMB_ERR_ARG, "close-pair rule %d order %d",
+  pl->nq = nq;
+  rc = fmbh_malloc((void **)&pl->d_quad, sizeof(double) * 4 * nq);
+  if (!rc) rc = fmbh_h2d(pl->d_quad, q, sizeof(double) * 4 * nq, stream);
+  if (!rc) rc = fmbh_sync(stream);
+  free(q);
+  if (rc) return rc;
+  int nqn = fmbx_tri_rule(prm->near_rule, prm->near_order, &q);
+  if (nqn < 1)
+    return fmb_set_error(FMB_ERR_ARG, "close-pair rule %d order %d",
                          prm->near_rule, prm->near_order);
-      goto fail;
+  pl->nqn = nqn;
+  rc = fmbh_malloc((void **)&pl->d_quadn, sizeof(double) * 4 * nqn);
+  if (!rc) rc = fmbh_h2d(pl->d_quadn, q, sizeof(double) * 4 * nqn, stream);
+  if (!rc) rc = fmbh_sync(stream);
+  free(q);
+  if (rc) return rc;
+  int ne = prm->nedge;
+  double *gx = (double *)malloc(sizeof(double) * ne),
+         *gw = (double *)malloc(sizeof(double) * ne),
+         *e = (double *)malloc(sizeof(double) * 2 * ne);
+  fmbx_gauss_legendre(ne, gx, gw);
+  for (int i = 0; i < ne; i++) {
+    e[2 * i] = gx[i];
+    e[2 * i + 1] = gw[i];
+  }
+  rc = fmbh_malloc((void **)&pl->d_edge, sizeof(double) * 2 * ne);
+  if (!rc) rc = fmbh_h2d(pl->d_edge, e, sizeof(double) * 2 * ne, stream);
+  if (!rc) rc = fmbh_sync(stream);
+  free(gx);
+  free(gw);
+  free(e);
+  if (rc) return rc;
+  return fmbh_near_layout(pl, stream);
+}
+
+/* rotation blocks of every polar angle at order pl->Pmax, generated in FP64
+ * and converted per opset, with the padded blocks and phase tables of the
+ * matrix-core translations */
+static int plan_rot(fmb_plan *pl, void *stream) {
+  size_t b0 = pl->bytes;
+  size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
+  double betas[FMB_NOFF], *d_H64 = NULL;
+  for (int i = 0; i < pl->nbeta; i++) betas[i] = acos(pl->beta_cos[i]);
+  int rc = fmbh_malloc((void **)&d_H64, sizeof(double) * hs);
+  if (!rc) rc = fmbh_gen_H(betas, pl->nbeta, pl->Pmax, d_H64, stream);
+  for (int i = 0; i < pl->nops && !rc; i++) {
+    fmb_opset *o = &pl->ops[i];
+    size_t bytes = o->csz / 2 * hs;
+    rc = fmbh_malloc(&o->d_H, bytes);
+    if (rc) break;
+    pl->bytes += bytes;
+    rc = fmbh_convert_real(o->prec, d_H64, hs, o->d_H, stream);
+  }
+  if (!rc) rc = fmbh_sync(stream);
+  fmbh_free(d_H64);
+  if (!rc) rc = fmbx_rcr_rot_setup(pl, stream);
+  pl->bytes_rot += pl->bytes - b0;
+  return rc;
+}
+
+/* operators of one precision from the FP64 coaxial originals */
+static int build_opset(fmb_plan *pl, fmb_opset *o, fmb_c64 *const *d_sr64,
+                       fmb_c64 *const *d_rr64, void *st) {
+  const fmb_tree *t = pl->tree;
+  int rc, L = t->lmax;
+  for (int l = 2; l <= L; l++) {
+    int p = pl->p[l];
+    size_t cs = (size_t)fmb_coax_size(p);
+    for (int d = 0; d < FMB_NDIST; d++) {
+      /* M2L: L~ = w L, M = w M~  ->  diag(w) (S|R) diag(w) */
+      if ((rc = fmbh_coax_scale(o->prec, d_sr64[l] + cs * d, p, pl->d_scale[l],
+                                pl->d_scale[l], 0, 0,
+                                (char *)o->d_Csr[l] + cs * d * o->csz, st)))
+        return rc;
     }
-    pl->nqn = nqn;
-    rc = fmbh_malloc((void **)&pl->d_quadn, sizeof(double) * 4 * nqn);
-    if (!rc) rc = fmbh_h2d(pl->d_quadn, q, sizeof(double) * 4 * nqn, stream);
-    free(q);
-    if (rc) goto fail;
-    int ne = prm->nedge;
-    double *gx = (double *)malloc(sizeof(double) * ne),
-           *gw = (double *)malloc(sizeof(double) * ne),
-           *e = (double *)malloc(sizeof(double) * 2 * ne);
-    fmbx_gauss_legendre(ne, gx, gw);
-    for (int i = 0; i < ne; i++) {
-      e[2 * i] = gx[i];
-      e[2 * i + 1] = gw[i];
+    if (l > 2) {
+      int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
+      /* M2M child l -> parent l-1: diag(1/w_{l-1}) (R|R) diag(w_l) */
+      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l - 1],
+                                pl->d_scale[l], 1, 0, o->d_Cm2m[l], st)))
+        return rc;
+      /* L2L parent l-1 -> child l: diag(w_l) (R|R) diag(1/w_{l-1}) */
+      if ((rc = fmbh_coax_scale(o->prec, d_rr64[l], P, pl->d_scale[l],
+                                pl->d_scale[l - 1], 0, 1, o->d_Cl2l[l], st)))
+        return rc;
     }
-    rc = fmbh_malloc((void **)&pl->d_edge, sizeof(double) * 2 * ne);
-    if (!rc) rc = fmbh_h2d(pl->d_edge, e, sizeof(double) * 2 * ne, stream);
-    free(gx);
-    free(gw);
-    free(e);
-    if (rc) goto fail;
+  }
+  int op = (int)(o - pl->ops);
+  if ((rc = fmbh_bases(pl, op, pl->d_robin, st))) return rc;
+  if ((rc = fmbh_near_assemble(pl, op, pl->d_robin, st))) return rc;
+  return fmbh_sync(st);
+}
+
+/* everything that depends on the wavenumber; the rotation tier is rebuilt
+ * first when an order exceeds the current Pmax */
+static int plan_freq(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream) {
+  const fmb_tree *t = pl->tree;
+  const fmb_plan_params *prm = &pl->prm;
+  int rc, L = t->lmax, P, p[FMB_MAXLEV] = {0};
+  if ((rc = level_orders(pl, k, p, &P))) return rc;
+  if (prm->k_max > k) {
+    int pk[FMB_MAXLEV], Pk;
+    if ((rc = level_orders(pl, prm->k_max, pk, &Pk))) return rc;
+    if (Pk > P) P = Pk;
+  }
+  if (P > pl->Pmax) {
+    plan_rot_free(pl);
+    pl->Pmax = P;
+    if ((rc = plan_rot(pl, stream))) return rc;
+  }
+  size_t b0 = pl->bytes;
+  fmb_c64 *d_sr64[FMB_MAXLEV] = {0}, *d_rr64[FMB_MAXLEV] = {0};
+  pl->k = k;
+  if (prm->alpha_re == 0.0 && prm->alpha_im == 0.0) {
+    pl->alpha_re = 0.0;
+    pl->alpha_im = 1.0 / k; /* alpha = i/k */
+  } else {
+    pl->alpha_re = prm->alpha_re;
+    pl->alpha_im = prm->alpha_im;
+  }
+  memcpy(pl->p, p, sizeof(p));
+  /* scale factors w_n = min(1, 1/(x |h_n(x)|)), x = k * box radius */
+  for (int l = 2; l <= L; l++) {
+    double x = k * t->D * ldexp(1.0, -l) * sqrt(3.0) / 2.0;
+    int n1 = pl->Pmax + 1;
+    pl->h_scale[l] = (double *)malloc(sizeof(double) * n1);
+    fmbx_abs_hn(n1 - 1, x, pl->h_scale[l]);
+    for (int n = 0; n < n1; n++) {
+      double w = 1.0 / (x * pl->h_scale[l][n]);
+      if (!(w > 1e-280)) w = 1e-280;
+      pl->h_scale[l][n] = w < 1.0 ? w : 1.0;
+    }
+    if ((rc = fmbh_malloc((void **)&pl->d_scale[l], sizeof(double) * n1))) goto out;
+    if ((rc = fmbh_h2d(pl->d_scale[l], pl->h_scale[l], sizeof(double) * n1, stream)))
+      goto out;
+    if ((rc = fmbh_mmajor_table(pl->p[l], &pl->d_tab_mmajor[l], stream))) goto out;
   }
   /* Robin coefficients in Morton order */
   if (robin) {
@@ -345,65 +419,89 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     if (!rc) rc = fmbh_d2h(perm, t->d_perm, sizeof(int) * N, stream);
     if (!rc)
       for (int i = 0; i < N; i++) hs[i] = h[perm[i]];
-    if (!rc) rc = fmbh_malloc((void **)&d_robin, sizeof(fmb_c64) * N);
-    if (!rc) rc = fmbh_h2d(d_robin, hs, sizeof(fmb_c64) * N, stream);
+    if (!rc) rc = fmbh_malloc((void **)&pl->d_robin, sizeof(fmb_c64) * N);
+    if (!rc) rc = fmbh_h2d(pl->d_robin, hs, sizeof(fmb_c64) * N, stream);
+    if (!rc) rc = fmbh_sync(stream);
     free(h);
     free(hs);
     free(perm);
-    if (rc) goto fail;
+    if (rc) goto out;
   }
-  if ((rc = fmbh_near_layout(pl, stream))) goto fail;
-  /* FP64 originals */
-  {
-    size_t hs = (size_t)fmb_rot_size(pl->Pmax) * pl->nbeta;
-    double betas[FMB_NOFF];
-    for (int i = 0; i < pl->nbeta; i++) betas[i] = acos(pl->beta_cos[i]);
-    if ((rc = fmbh_malloc((void **)&d_H64, sizeof(double) * hs))) goto fail;
-    if ((rc = fmbh_gen_H(betas, pl->nbeta, pl->Pmax, d_H64, stream))) goto fail;
-    for (int l = 2; l <= L; l++) {
-      double side = t->D * ldexp(1.0, -l), tv[FMB_NDIST];
-      for (int d = 0; d < FMB_NDIST; d++) tv[d] = side * sqrt((double)k_dist2[d]);
-      size_t cs = (size_t)fmb_coax_size(pl->p[l]) * FMB_NDIST;
-      if ((rc = fmbh_malloc((void **)&d_sr64[l], sizeof(fmb_c64) * cs))) goto fail;
-      if ((rc = fmbh_gen_coax(FMB_COAX_SR, pl->k, tv, FMB_NDIST, pl->p[l],
-                              d_sr64[l], stream)))
-        goto fail;
-      if (l > 2) {
-        int P = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
-        double r = side * sqrt(3.0) / 2.0;
-        if ((rc = fmbh_malloc((void **)&d_rr64[l],
-                              sizeof(fmb_c64) * fmb_coax_size(P))))
-          goto fail;
-        if ((rc = fmbh_gen_coax(FMB_COAX_RR, pl->k, &r, 1, P, d_rr64[l], stream)))
-          goto fail;
-      }
+  /* FP64 coaxial originals at this wavenumber */
+  for (int l = 2; l <= L; l++) {
+    double side = t->D * ldexp(1.0, -l), tv[FMB_NDIST];
+    for (int d = 0; d < FMB_NDIST; d++) tv[d] = side * sqrt((double)k_dist2[d]);
+    size_t cs = (size_t)fmb_coax_size(pl->p[l]) * FMB_NDIST;
+    if ((rc = fmbh_malloc((void **)&d_sr64[l], sizeof(fmb_c64) * cs))) goto out;
+    if ((rc = fmbh_gen_coax(FMB_COAX_SR, k, tv, FMB_NDIST, pl->p[l], d_sr64[l], stream)))
+      goto out;
+    if (l > 2) {
+      int Pc = pl->p[l] > pl->p[l - 1] ? pl->p[l] : pl->p[l - 1];
+      double r = side * sqrt(3.0) / 2.0;
+      if ((rc = fmbh_malloc((void **)&d_rr64[l], sizeof(fmb_c64) * fmb_coax_size(Pc))))
+        goto out;
+      if ((rc = fmbh_gen_coax(FMB_COAX_RR, k, &r, 1, Pc, d_rr64[l], stream))) goto out;
     }
   }
+  for (int i = 0; i < pl->nops; i++) {
+    if ((rc = opset_alloc(pl, &pl->ops[i]))) goto out;
+    if ((rc = build_opset(pl, &pl->ops[i], d_sr64, d_rr64, stream))) goto out;
+  }
+  rc = fmbx_rcr_setup(pl, stream);
+out:
+  for (int l = 0; l < FMB_MAXLEV; l++) {
+    fmbh_free(d_sr64[l]);
+    fmbh_free(d_rr64[l]);
+  }
+  pl->bytes_freq += pl->bytes - b0;
+  if (!rc) pl->ready = 1;
+  return rc;
+}
+
+int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
+                    const fmb_c64 *robin, void *stream, fmb_plan **out) {
+  if (!t || !prm || !out || !(prm->k > 0) || prm->nrhs_max < 1 ||
+      (prm->precision != FMB_FP64 && prm->precision != FMB_MIXED) ||
+      !(prm->k_max >= 0))
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: invalid parameters");
+  if (prm->nedge < 1 || prm->nedge > 256 || prm->near_ratio < 0)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: quadrature parameters");
+  if (prm->rcr_path != FMB_RCR_AUTO && prm->rcr_path != FMB_RCR_VALU)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
+  int rc;
+  fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
+  pl->tree = t;
+  pl->prm = *prm;
+  pl->stream = stream;
+  pl->nrhs_max = prm->nrhs_max;
   pl->nops = prm->precision == FMB_MIXED ? 2 : 1;
   pl->work_prec = prm->precision;
   for (int i = 0; i < pl->nops; i++) {
-    if ((rc = opset_alloc(pl, &pl->ops[i], i == 0 ? FMB_FP64 : FMB_MIXED))) goto fail;
-    if ((rc = build_opset(pl, &pl->ops[i], d_H64, d_sr64, d_rr64, d_robin)))
-      goto fail;
+    pl->ops[i].prec = i == 0 ? FMB_FP64 : FMB_MIXED;
+    pl->ops[i].csz = i == 0 ? 16 : 8;
   }
-  if ((rc = fmbx_rcr_setup(pl, stream))) goto fail;
-  fmbh_free(d_H64);
-  for (int l = 0; l < FMB_MAXLEV; l++) {
-    fmbh_free(d_sr64[l]);
-    fmbh_free(d_rr64[l]);
+  if ((rc = plan_geometry(pl, stream)) || (rc = plan_freq(pl, prm->k, robin, stream))) {
+    fmb_plan_destroy(pl);
+    return rc;
   }
-  fmbh_free(d_robin);
   *out = pl;
   return FMB_OK;
-fail:
-  fmbh_free(d_H64);
-  for (int l = 0; l < FMB_MAXLEV; l++) {
-    fmbh_free(d_sr64[l]);
-    fmbh_free(d_rr64[l]);
-  }
-  fmbh_free(d_robin);
-  fmb_plan_destroy(pl);
+}
+
+int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream) {
+  if (!pl || !(k > 0))
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_retune: invalid arguments");
+  pl->stream = stream;
+  plan_freq_free(pl);
+  int rc = plan_freq(pl, k, robin, stream);
+  if (rc) plan_freq_free(pl);
   return rc;
+}
+
+int fmb_plan_wavenumber(const fmb_plan *pl, double *k) {
+  if (!pl || !k) return fmb_set_error(FMB_ERR_ARG, "null plan");
+  *k = pl->ready ? pl->k : 0.0;
+  return FMB_OK;
 }
 
 int fmb_plan_orders(const fmb_plan *pl, int *p16) {
@@ -452,6 +550,7 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
   if (!pl || parts < 1 || parts > 3 || !x || !y || nrhs < 1 || nrhs > pl->nrhs_max)
     return fmb_set_error(FMB_ERR_ARG, "fmb_matvec: nrhs %d (max %d)", nrhs,
                          pl ? pl->nrhs_max : 0);
+  if (!pl->ready) return fmb_set_error(FMB_ERR_ARG, "fmb_matvec: plan holds no operator");
   int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
   const fmb_tree *t = pl->tree;
   fmb_opset *o = &pl->ops[op];
@@ -512,7 +611,31 @@ out:
 
 int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,
                        fmb_c64 *b, void *stream) {
-  if (!pl || !dirs || !b || nrhs < 1)
+  if (!pl || !pl->ready || !dirs || !b || nrhs < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_rhs_plane_wave: invalid arguments");
   return fmbh_rhs_plane(pl, dirs, nrhs, b, stream);
+}
+
+int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
+                   const double *pts, int npts, const double *dirs, fmb_c64 *out,
+                   void *stream) {
+  if (!pl || !pl->ready || !x || nrhs < 1 || !pts || npts < 1 || !out)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_field_eval: invalid arguments");
+  double *d_pts = NULL, *d_dirs = NULL;
+  int rc = fmbh_malloc((void **)&d_pts, sizeof(double) * 3 * (size_t)npts);
+  if (!rc) rc = fmbh_h2d(d_pts, pts, sizeof(double) * 3 * (size_t)npts, stream);
+  if (!rc && dirs) rc = fmbh_malloc((void **)&d_dirs, sizeof(double) * 3 * (size_t)nrhs);
+  if (!rc && dirs) rc = fmbh_h2d(d_dirs, dirs, sizeof(double) * 3 * (size_t)nrhs, stream);
+  if (!rc) rc = fmbh_field(pl, x, nrhs, d_pts, npts, d_dirs, out, stream);
+  if (!rc) rc = fmbh_sync(stream);
+  fmbh_free(d_pts);
+  fmbh_free(d_dirs);
+  return rc;
+}
+
+int fmb_sweep_guess(const fmb_plan *pl, double k_from, const fmb_c64 *x_from,
+                    const double *dirs, int nrhs, fmb_c64 *x0, void *stream) {
+  if (!pl || !pl->ready || !x_from || !x0 || nrhs < 1 || !(k_from > 0))
+    return fmb_set_error(FMB_ERR_ARG, "fmb_sweep_guess: invalid arguments");
+  return fmbh_sweep_guess(pl, pl->k - k_from, x_from, dirs, nrhs, x0, stream);
 }

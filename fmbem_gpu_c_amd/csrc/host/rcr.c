@@ -244,9 +244,64 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
   return rc;
 }
 
+int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
+  if (pl->prm.rcr_path == FMB_RCR_VALU) return FMB_OK;
+  int rc = FMB_OK;
+  /* e^{i m phi} of every bucket, m < Pmax, per opset */
+  int nph[3] = {FMB_NOFF, 8, 8};
+  for (int q = 0; q < 3 && !rc; q++) {
+    double *ph = (double *)malloc(sizeof(double) * 2 * nph[q] * pl->Pmax);
+    for (int b = 0; b < nph[q]; b++) {
+      double phi = q == 0 ? pl->off_phi[b] : (q == 1 ? pl->m2m_phi[b] : pl->l2l_phi[b]);
+      for (int m = 0; m < pl->Pmax; m++) {
+        ph[2 * ((size_t)b * pl->Pmax + m)] = cos(m * phi);
+        ph[2 * ((size_t)b * pl->Pmax + m) + 1] = sin(m * phi);
+      }
+    }
+    for (int op = 0; op < pl->nops && !rc; op++) {
+      fmb_opset *o = &pl->ops[op];
+      size_t n = (size_t)2 * nph[q] * pl->Pmax;
+      rc = fmbh_malloc(&o->d_phase[q], o->csz / 2 * n);
+      if (rc) break;
+      if (o->prec == FMB_FP64) {
+        rc = fmbh_h2d(o->d_phase[q], ph, sizeof(double) * n, stream);
+        if (!rc) rc = fmbh_sync(stream);
+      } else {
+        float *f = (float *)malloc(sizeof(float) * n);
+        for (size_t i = 0; i < n; i++) f[i] = (float)ph[i];
+        rc = fmbh_h2d(o->d_phase[q], f, sizeof(float) * n, stream);
+        if (!rc) rc = fmbh_sync(stream);
+        free(f);
+      }
+    }
+    free(ph);
+  }
+  if (rc) return rc;
+  /* padded rotation blocks of every polar angle, per opset */
+  int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1)), *d_hoff = NULL;
+  rot_offsets(pl->Pmax, hoff);
+  rc = fmbh_malloc((void **)&d_hoff, sizeof(int) * (pl->Pmax + 1));
+  if (!rc) rc = fmbh_h2d(d_hoff, hoff, sizeof(int) * (pl->Pmax + 1), stream);
+  for (int op = 0; op < pl->nops && !rc; op++) {
+    fmb_opset *o = &pl->ops[op];
+    o->hpad_stride = (size_t)hoff[pl->Pmax];
+    size_t bytes = o->csz / 2 * o->hpad_stride * pl->nbeta;
+    rc = fmbh_malloc(&o->d_Hpad, bytes);
+    if (!rc) {
+      pl->bytes += bytes;
+      rc = fmbh_pad_rot(o->prec, o->d_H, pl->nbeta, pl->Pmax, d_hoff, o->hpad_stride,
+                        o->d_Hpad, stream);
+    }
+  }
+  if (!rc) rc = fmbh_sync(stream);
+  fmbh_free(d_hoff);
+  free(hoff);
+  return rc;
+}
+
 int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
   const fmb_tree *t = pl->tree;
-  int L = t->lmax, rc;
+  int L = t->lmax, rc = FMB_OK;
   for (int kind = 1; kind <= 2; kind++) {
     rcr_info_t info[8];
     for (int o = 0; o < 8; o++) {
@@ -256,54 +311,11 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     }
     if ((rc = fmbh_malloc(&pl->d_info_oct[kind - 1], sizeof(info)))) return rc;
     if ((rc = fmbh_h2d(pl->d_info_oct[kind - 1], info, sizeof(info), stream))) return rc;
+    if ((rc = fmbh_sync(stream))) return rc;
   }
   if (pl->prm.rcr_path != FMB_RCR_VALU) {
-    /* e^{i m phi} of every bucket, m < Pmax, per opset */
-    int nph[3] = {FMB_NOFF, 8, 8};
-    for (int q = 0; q < 3 && !rc; q++) {
-      double *ph = (double *)malloc(sizeof(double) * 2 * nph[q] * pl->Pmax);
-      for (int b = 0; b < nph[q]; b++) {
-        double phi = q == 0 ? pl->off_phi[b] : (q == 1 ? pl->m2m_phi[b] : pl->l2l_phi[b]);
-        for (int m = 0; m < pl->Pmax; m++) {
-          ph[2 * ((size_t)b * pl->Pmax + m)] = cos(m * phi);
-          ph[2 * ((size_t)b * pl->Pmax + m) + 1] = sin(m * phi);
-        }
-      }
-      for (int op = 0; op < pl->nops && !rc; op++) {
-        fmb_opset *o = &pl->ops[op];
-        size_t n = (size_t)2 * nph[q] * pl->Pmax;
-        rc = fmbh_malloc(&o->d_phase[q], o->csz / 2 * n);
-        if (rc) break;
-        if (o->prec == FMB_FP64) {
-          rc = fmbh_h2d(o->d_phase[q], ph, sizeof(double) * n, stream);
-        } else {
-          float *f = (float *)malloc(sizeof(float) * n);
-          for (size_t i = 0; i < n; i++) f[i] = (float)ph[i];
-          rc = fmbh_h2d(o->d_phase[q], f, sizeof(float) * n, stream);
-          free(f);
-        }
-      }
-      free(ph);
-    }
-    if (rc) return rc;
-    /* padded rotation blocks of every polar angle, per opset */
-    int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1)), *d_hoff = NULL;
+    int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
     rot_offsets(pl->Pmax, hoff);
-    rc = fmbh_malloc((void **)&d_hoff, sizeof(int) * (pl->Pmax + 1));
-    if (!rc) rc = fmbh_h2d(d_hoff, hoff, sizeof(int) * (pl->Pmax + 1), stream);
-    for (int op = 0; op < pl->nops && !rc; op++) {
-      fmb_opset *o = &pl->ops[op];
-      o->hpad_stride = (size_t)hoff[pl->Pmax];
-      size_t bytes = o->csz / 2 * o->hpad_stride * pl->nbeta;
-      rc = fmbh_malloc(&o->d_Hpad, bytes);
-      if (!rc) {
-        pl->bytes += bytes;
-        rc = fmbh_pad_rot(o->prec, o->d_H, pl->nbeta, pl->Pmax, d_hoff, o->hpad_stride,
-                          o->d_Hpad, stream);
-      }
-    }
-    if (!rc) rc = fmbh_sync(stream);
-    fmbh_free(d_hoff);
     for (int l = 2; l <= L && !rc; l++) {
       fmb_rcr_tab *tb = &pl->rcr[0][l];
       void *C[2] = {pl->ops[0].d_Csr[l], pl->ops[1].d_Csr[l]};

@@ -146,6 +146,10 @@ typedef struct fmb_plan_params {
   int nedge;                 /* Gauss points per edge, singular self terms  */
   int nrhs_max;              /* right-hand sides per matvec                 */
   int rcr_path;              /* FMB_RCR_*: translation kernel family        */
+  double k_max;              /* largest wavenumber of a frequency sweep: the
+                                rotation tables are built for its orders
+                                once (initOctree's maxWavNum,
+                                translation.cu:2992); 0: k                  */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);
@@ -158,6 +162,20 @@ void fmb_plan_params_default(fmb_plan_params *prm);
 int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
                     const fmb_c64 *robin, void *stream, fmb_plan **out);
 int fmb_plan_orders(const fmb_plan *pl, int *p16);
+
+/* genOctree called again on a filled octree (translation.cu:3007-3186,
+ * 3294-3302): re-targets the plan at wavenumber k for a frequency sweep.
+ * Kept: the tree, translation directions, quadrature rules, near-field layout
+ * and the rotation tables (rebuilt only when an order exceeds the largest
+ * built so far, or fmb_plan_params.k_max's).  Rebuilt: orders, scale
+ * factors, coaxial matrices, P2M/L2P bases, near-field blocks and translation
+ * schedules; alpha follows the creation parameters again (0,0: i/k).  robin
+ * as for fmb_plan_create, at the new frequency.  The operator equals that of
+ * fmb_plan_create at k with the same parameters.  On failure the plan holds
+ * no operator: only fmb_plan_retune and fmb_plan_destroy accept it. */
+int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream);
+/* Wavenumber of the operator the plan holds (0 after a failed retune). */
+int fmb_plan_wavenumber(const fmb_plan *pl, double *k);
 /* Device bytes held by the plan. */
 int fmb_plan_memory(const fmb_plan *pl, double *bytes);
 
@@ -171,6 +189,20 @@ int fmb_matvec(fmb_plan *pl, int prec, const fmb_c64 *x, fmb_c64 *y,
  * (P2M .. L2P: the part the reference's OprL computes), 3 both. */
 int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
                      fmb_c64 *y, int nrhs, void *stream);
+
+/* Pressure at field points off the surface from a surface solution x
+ * ([nrhs][N] device, caller order): the exterior representation formula
+ * p(y) = p_inc(y) + sum_j x_j int_j (dG/dn_y - sigma_j G) dS_y with the
+ * plan's kernels, signs and panel rules (the close-panel rule when
+ * |y - c_j| < near_ratio * radius_j).  pts: npts*3 host doubles; dirs:
+ * nrhs*3 host doubles (unit plane waves added as the incident field) or
+ * NULL for the scattered field; out: [nrhs][npts] device.  FP64, direct
+ * O(N npts) summation.  The reference has none (its OprL stops at the
+ * surface); it serves field-point objectives such as the sum of |p|^2
+ * on a circle of points. */
+int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
+                   const double *pts, int npts, const double *dirs, fmb_c64 *out,
+                   void *stream);
 
 /* Right-hand side p_inc + alpha dp_inc/dn for unit plane waves e^{ik d.x}
  * (dirs: nrhs*3 host doubles).  The reference has none. */
@@ -193,6 +225,15 @@ typedef struct fmb_gmres_info {
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream);
+/* Initial guess for a sweep step from the solution x_from at wavenumber
+ * k_from: x0_j = x_from_j e^{i (k - k_from) d_r . c_j} with c_j the centroid
+ * of element j and d_r the unit plane-wave direction of right-hand side r
+ * (dirs: nrhs*3 host doubles; NULL gives x0 = x_from), k the plan's
+ * wavenumber.  It carries the incident phase across the step, so that
+ * the illuminated part of the solution stays in place.  x_from and x0 may
+ * alias.  The reference has no solver. */
+int fmb_sweep_guess(const fmb_plan *pl, double k_from, const fmb_c64 *x_from,
+                    const double *dirs, int nrhs, fmb_c64 *x0, void *stream);
 /* Precision of the Arnoldi matvecs of fmb_gmres on this plan (FMB_MIXED only
  * on a plan created with FMB_MIXED).  The reference has no solver. */
 int fmb_plan_set_work_precision(fmb_plan *pl, int prec);

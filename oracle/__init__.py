@@ -76,6 +76,7 @@ _SIGS = {
     "fo_bem_fmm_orders": (_ci, [_vp, _ip]),
     "fo_bem_fmm_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_bem_fmm_time_sample": (ctypes.c_double, [_vp, _ci, _dp]),
+    "fo_bem_field": (None, [_vp, _ci, _dp, _dp, _dp, _ci, _vp, _dp, _dp]),
     "fo_bem_near_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_gmres": (_ci, [_vp, _ci, _ci, _cd, _ci, _dp, _dp, _dp, _dp, ctypes.POINTER(_ci),
                        ctypes.POINTER(_cd)]),
@@ -356,6 +357,23 @@ class Bem:
 
     def near_matvec(self, x):
         return self._mv(lib().fo_bem_near_matvec, x)
+
+    def field(self, x, pts, dirs=None):
+        """Pressure at field points pts (M x 3) from the surface solution x
+        (nrhs x N or N); dirs: incident plane-wave directions (nrhs x 3) or
+        None for the scattered field only."""
+        x = np.atleast_2d(np.asarray(x, dtype=np.complex128))
+        nrhs = x.shape[0]
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+        ore = np.zeros((nrhs, pts.shape[0]))
+        oim = np.zeros((nrhs, pts.shape[0]))
+        d = None
+        if dirs is not None:
+            d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+        lib().fo_bem_field(self.h, nrhs, xr, xi, pts, pts.shape[0],
+                           d.ctypes.data_as(ctypes.c_void_p) if d is not None else None, ore, oim)
+        return ore + 1j * oim
 
     def fmm_time_sample(self, stride):
         """Estimated seconds of one FMM matvec (see fo_bem_fmm_time_sample)

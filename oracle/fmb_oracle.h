@@ -126,6 +126,12 @@ void fo_bem_dense_matvec(const fo_bem *b, int nrhs, const double *xre,
 void fo_bem_rhs_plane(const fo_bem *b, const double *dir, double *re,
                       double *im);
 void fo_bem_centroids(const fo_bem *b, double *c, double *nrm, double *area);
+/* Pressure at field points (exterior representation formula): x [nrhs][N]
+ * surface solution; pts npts*3; dirs nrhs*3 incident plane waves or NULL;
+ * out [nrhs][npts]. */
+void fo_bem_field(const fo_bem *b, int nrhs, const double *xre, const double *xim,
+                  const double *pts, int npts, const double *dirs, double *ore,
+                  double *oim);
 
 int fo_bem_fmm_setup(fo_bem *b, int s, int depth, const int *plev);
 int fo_bem_fmm_orders(const fo_bem *b, int *p);

@@ -194,6 +194,60 @@ static fo_cplx entry(const fo_bem *b, int i, int j) {
   return v;
 }
 
+/* Pressure at field points off the surface by the exterior representation
+ * formula p(y) = p_inc(y) + sum_j x_j int_j (dG/dn_y - sigma_j G) dS_y, the
+ * panel rule of the matvec (refined when |y - c_j| < ratio * radius_j).
+ * The same kernels and signs as the collocation entries above. */
+void fo_bem_field(const fo_bem *b, int nrhs, const double *xre, const double *xim,
+                  const double *pts, int npts, const double *dirs, double *ore,
+                  double *oim) {
+  int N = b->N;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < npts; q++) {
+    const double *yq = pts + 3 * q;
+    fo_cplx *acc = (fo_cplx *)calloc(nrhs, sizeof(fo_cplx));
+    double zero[3] = {0, 0, 0};
+    for (int j = 0; j < N; j++) {
+      const double *t = b->tri + 9 * j;
+      const double *bary = b->bary, *wq = b->wq;
+      int nq = b->nq;
+      double d2 = 0.0;
+      for (int c = 0; c < 3; c++) {
+        double d = yq[c] - b->ctr[3 * j + c];
+        d2 += d * d;
+      }
+      if (b->nqn > 0 && d2 < b->near_ratio * b->near_ratio * b->rad[j] * b->rad[j]) {
+        bary = b->baryn;
+        wq = b->wqn;
+        nq = b->nqn;
+      }
+      fo_cplx S = 0.0, K = 0.0;
+      for (int g = 0; g < nq; g++) {
+        double y[3], kr[4], ki[4];
+        for (int c = 0; c < 3; c++)
+          y[c] = bary[3 * g] * t[c] + bary[3 * g + 1] * t[3 + c] + bary[3 * g + 2] * t[6 + c];
+        fo_kernels(b->k, yq, zero, y, b->nrm + 3 * j, kr, ki);
+        double w = wq[g] * b->area[j];
+        S += w * (kr[0] + I * ki[0]);
+        K += w * (kr[1] + I * ki[1]);
+      }
+      fo_cplx gj = K - b->sig[j] * S;
+      for (int r = 0; r < nrhs; r++)
+        acc[r] += gj * (xre[(size_t)r * N + j] + I * xim[(size_t)r * N + j]);
+    }
+    for (int r = 0; r < nrhs; r++) {
+      fo_cplx v = acc[r];
+      if (dirs) {
+        const double *d = dirs + 3 * r;
+        v += cexp(I * b->k * (d[0] * yq[0] + d[1] * yq[1] + d[2] * yq[2]));
+      }
+      ore[(size_t)r * npts + q] = creal(v);
+      oim[(size_t)r * npts + q] = cimag(v);
+    }
+    free(acc);
+  }
+}
+
 void fo_bem_dense_row(const fo_bem *b, int i, double *re, double *im) {
   for (int j = 0; j < b->N; j++) {
     fo_cplx v = entry(b, i, j);

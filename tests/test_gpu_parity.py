@@ -188,3 +188,33 @@ def test_rcr_matrix_cores_match_valu(freq, orders):
           % (freq, orders[2:], e64, e32, e32_64), flush=True)
     assert e64 < 1e-12
     assert e32 < 1e-5
+
+
+@pytest.mark.parametrize("robin", [False, True])
+def test_field_points_match_oracle(sphere_l3, robin):
+    """Pressure at 32 points on an r = 0.5 m circle plus 8 points just off the
+    surface (close-panel rule) from random surface data, device (FP64
+    direct sum) against the oracle's representation formula; rigid and
+    Robin (sigma = i k Y) surfaces, incident plane waves added."""
+    m, V, Fc = sphere_l3
+    k = wavenumber(1000.0)
+    n = Fc.shape[0]
+    rng = np.random.default_rng(21)
+    sig = None
+    if robin:
+        Y = 0.01 * (1 + 0.5 * rng.uniform(-1, 1, n)) + 0.005j * rng.uniform(-1, 1, n)
+        sig = 1j * k * Y
+    t = F.Tree(m, s=5)
+    pl = F.Plan(t, k, robin=None if sig is None else torch.tensor(sig, device="cuda"), nrhs_max=2)
+    ob = O.Bem(V, Fc, k, sigma=sig)
+    th = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    pts = np.stack([0.5 * np.sin(th), np.zeros_like(th), 0.5 * np.cos(th)], 1)
+    c = V[Fc].mean(1)[:8]
+    pts = np.vstack([pts, c * 1.02])
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    dirs = [[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]]
+    got = pl.field(torch.tensor(x, device="cuda"), pts, dirs).cpu().numpy()
+    want = ob.field(x, pts, dirs)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("field robin=%s rel err %.2e" % (robin, err), flush=True)
+    assert err < 1e-12

@@ -97,3 +97,44 @@ def test_gmres_matches_direct_solve(sphere2):
     assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-8
     # restarts happened (m=10) and the true residual is what was reported
     assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) == pytest.approx(rr, rel=1e-3)
+
+
+def mie_exterior_scattered(k, a, r, ct, nterms=40, sigma=0.0):
+    """Scattered pressure of a unit plane wave e^{ikz} on a sphere of radius a
+    with dp/dr = sigma p (sigma = 0: rigid) at distance r, angle acos(ct):
+    sum (2n+1) i^n c_n h_n(kr) P_n(ct), h_n = j_n + i y_n (e^{ikr}), with
+    c_n = -(k j_n'(ka) - sigma j_n(ka)) / (k h_n'(ka) - sigma h_n(ka))."""
+    n = np.arange(nterms + 1)
+    ja, jpa = sp.spherical_jn(n, k * a), sp.spherical_jn(n, k * a, derivative=True)
+    ha = ja + 1j * sp.spherical_yn(n, k * a)
+    hp = jpa + 1j * sp.spherical_yn(n, k * a, derivative=True)
+    h = sp.spherical_jn(n, k * r) + 1j * sp.spherical_yn(n, k * r)
+    c = (2 * n + 1) * (1j ** n) * (-(k * jpa - sigma * ja) / (k * hp - sigma * ha))
+    P = np.array([sp.eval_legendre(q, ct) for q in n])
+    return ((c * h)[:, None] * P).sum(0)
+
+
+@pytest.mark.parametrize("Y", [0.0, 0.3 + 0.2j])
+def test_field_points_converge_to_exterior_mie(sphere2, sphere3, Y):
+    """The representation formula at 32 points on an r = 0.5 m circle (the
+    differentiation config's objective points) from the dense solve on a
+    rigid (Y = 0) or impedance sphere (dp/dn = ikY p, the sigma_j G term of
+    the formula): the scattered field approaches the exterior Mie series
+    under mesh refinement, and the incident wave is added exactly."""
+    k = wavenumber(1000.0)
+    th = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    pts = np.stack([0.5 * np.sin(th), np.zeros_like(th), 0.5 * np.cos(th)], 1)
+    exact = mie_exterior_scattered(k, 0.1, 0.5, np.cos(th), sigma=1j * k * Y)
+    errs = []
+    for V, Fc in (sphere2, sphere3):
+        sig = 1j * k * Y * np.ones(len(Fc))
+        b = O.Bem(V, Fc, k, sigma=sig if Y else None)
+        A = b.dense_matrix()
+        u = np.linalg.solve(A, b.rhs_plane([0.0, 0.0, 1.0]))
+        ps = b.field(u, pts)[0]
+        errs.append(np.linalg.norm(ps - exact) / np.linalg.norm(exact))
+        pt = b.field(u, pts, dirs=[[0.0, 0.0, 1.0]])[0]
+        np.testing.assert_allclose(pt - ps, np.exp(1j * k * pts[:, 2]), rtol=0, atol=1e-12)
+    print("field errors vs Mie", Y, errs)
+    assert errs[1] < 0.02, errs
+    assert errs[1] < 0.6 * errs[0], errs
