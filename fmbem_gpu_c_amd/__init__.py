@@ -104,6 +104,13 @@ _SIGS = {
     "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
     "fmb_field_eval": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "fmb_plan_retune": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp]),
+    "fmb_matvec_transpose": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                            ctypes.c_int, _vp]),
+    "fmb_gmres_transpose": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    "fmb_adjoint_grad": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp,
+                                        _vp, _vp, _vp]),
     "fmb_plan_wavenumber": (ctypes.c_int, [_vp, _vp]),
     "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -376,6 +383,51 @@ class Plan:
             _check(rc)
         return x, info
 
+    def matvec_t(self, x, parts=3, prec=FMB_FP64):
+        """y = A^T x (fmb_matvec_transpose)."""
+        import torch
+        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        y = torch.empty_like(x)
+        _check(lib().fmb_matvec_transpose(self.h, prec, parts, _dev(x), _dev(y), nrhs,
+                                          _stream()))
+        return y
+
+    def gmres_t(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
+        """Solves A^T x = b (fmb_gmres_transpose)."""
+        import torch
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        info = GmresInfo()
+        rc = lib().fmb_gmres_transpose(self.h, _dev(b), _dev(x), nrhs, restart, tol, maxit,
+                                       ctypes.byref(info), _stream())
+        if check:
+            _check(rc)
+        return x, info
+
+    def adjoint_grad(self, x, pts, dirs=None, restart=30, tol=1e-10, maxit=1000):
+        """J = sum |p|^2 at pts (M x 3) for the surface solution x and its
+        gradient dJ/dRe sigma + i dJ/dIm sigma per element (complex128 CUDA
+        tensor, caller order) with respect to the Robin coefficients
+        (fmb_adjoint_grad).  Returns (J, grad, info of the adjoint solve)."""
+        import torch
+        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        d = None
+        if dirs is not None:
+            d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+            if d.shape[0] != nrhs:
+                raise FmbError(-1, "one incident direction per right-hand side")
+        n = x.shape[-1]
+        g = torch.empty(n, dtype=torch.complex128, device=x.device)
+        J = ctypes.c_double()
+        info = GmresInfo()
+        _check(lib().fmb_adjoint_grad(self.h, _dev(x), nrhs,
+                                      ctypes.c_void_p(d.ctypes.data) if d is not None else None,
+                                      ctypes.c_void_p(pts.ctypes.data), pts.shape[0], restart,
+                                      tol, maxit, ctypes.byref(J), _dev(g), ctypes.byref(info),
+                                      _stream()))
+        return J.value, g, info
+
     def set_work_precision(self, prec):
         _check(lib().fmb_plan_set_work_precision(self.h, prec))
 
@@ -413,3 +465,36 @@ def gen_coax_mats(kind, k, t, p):
     _check(lib().fmb_gen_coax_mats(kind, k, _ptr(t, ctypes.c_double), t.shape[0], p,
                                    out.ctypes.data_as(ctypes.c_void_p)))
     return out
+
+
+def _admittance_function():
+    import torch
+
+    class AdmittanceObjective(torch.autograd.Function):
+        """J(Y) = sum_r sum_q |p_r(y_q)|^2 for plane waves dirs on a surface with
+        per-element admittance Y (Robin coefficient sigma = i k Y), solved on
+        `plan` (re-targeted at its own wavenumber with the new sigma).
+        backward returns dJ/dRe Y + i dJ/dIm Y from fmb_adjoint_grad, the
+        convention torch uses for a real loss of a complex tensor."""
+
+        @staticmethod
+        def forward(ctx, Y, plan, pts, dirs, tol):
+            k = plan.k
+            plan.retune(k, robin=(1j * k) * Y.detach())
+            b = plan.rhs_plane_wave(dirs)
+            x, _ = plan.gmres(b, tol=tol)
+            J, gs, _ = plan.adjoint_grad(x, pts, dirs, tol=tol)
+            ctx.save_for_backward((-1j * k) * gs)
+            return torch.tensor(J, dtype=torch.float64, device=Y.device)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (gY,) = ctx.saved_tensors
+            return gJ * gY, None, None, None, None
+
+    return AdmittanceObjective
+
+
+def admittance_objective(Y, plan, pts, dirs, tol=1e-10):
+    """Differentiable J(Y) (torch.autograd) through the adjoint FMM-BEM."""
+    return _admittance_function().apply(Y, plan, pts, dirs, tol)

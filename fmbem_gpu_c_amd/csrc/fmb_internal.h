@@ -64,6 +64,7 @@ typedef struct fmb_opset {
   void *d_gws;       /* global workspace of the GWS-mode translations       */
   void *d_Hpad;      /* zero-padded rotation blocks [nbeta][hpad_stride]    */
   void *d_phase[3];  /* e^{i m phi}, m < Pmax: M2L offsets, M2M, L2L octants */
+  void *d_phase_c[3]; /* e^{-i m phi}: the same buckets, transposed pass    */
   size_t hpad_stride;
   int gws_grid;      /* persistent workgroups sharing d_gws                  */
 } fmb_opset;

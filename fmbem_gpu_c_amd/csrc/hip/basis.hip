@@ -6,6 +6,8 @@
 //                                          triElemIntegral_R / _pRpn, 251-365)
 //   L2P_i[n,m] = [ R_n^m + alpha dR_n^m/dn_i ](x_i - c) / w_n
 // c is the leaf-box centre; dR/dn follows dirDev_R (integral.cu:284-331).
+// With dsig set the kernel writes dP2M_j/dsigma_j = ik sum_q w_q A_j R_n^{-m}
+// / w_n instead (and no L2P), the column factor of the sigma derivative.
 // One wavefront per element: lane 0 runs the serial Bessel/Legendre
 // diagonals, the lanes share the Legendre columns and the R table in LDS.
 #include "fmb_device.h"
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(WG)
             double D, double p0, double p1, double p2,
             const double2 *__restrict__ robin, const double *__restrict__ quad,
             int nq, const double *__restrict__ wsc, T *__restrict__ P2M,
-            T *__restrict__ L2P) {
+            T *__restrict__ L2P, int dsig) {
   extern __shared__ double sm[];
   int nm = pL;  // R needed up to degree pL for the derivative of degree pL-1
   int ntab = (nm + 1) * (nm + 2) / 2;
@@ -153,9 +155,9 @@ __global__ void __launch_bounds__(WG)
       int m = f - n * n - n;
       double2 R = Rt[n * n + n - m];
       double2 dR = dRdn(Rt, n, -m, k, nr);
-      // val = -dR + sigma R ; P2M += i k w val
-      double vr = -dR.x + sg.x * R.x - sg.y * R.y;
-      double vi = -dR.y + sg.x * R.y + sg.y * R.x;
+      // val = -dR + sigma R (dsig: R) ; P2M += i k w val
+      double vr = dsig ? R.x : -dR.x + sg.x * R.x - sg.y * R.y;
+      double vi = dsig ? R.y : -dR.y + sg.x * R.y + sg.y * R.x;
       acc[f].x += -k * w * vi;
       acc[f].y += k * w * vr;
     }
@@ -169,6 +171,7 @@ __global__ void __launch_bounds__(WG)
     P2M[2 * ((size_t)j * ps + f)] = (T)(acc[f].x * s);
     P2M[2 * ((size_t)j * ps + f) + 1] = (T)(acc[f].y * s);
   }
+  if (!L2P) return;
   // L2P at the centroid
   double v[3] = {g[0] - c[0], g[1] - c[1], g[2] - c[2]};
   r_table(nm, k, v, jn, diag, Pt, ph, Rt, g3);
@@ -189,8 +192,8 @@ __global__ void __launch_bounds__(WG)
 
 }  // namespace
 
-extern "C" int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *robin,
-                          void *stream) {
+static int launch_bases(fmb_plan *pl, int op, const fmb_c64 *robin, void *P2M,
+                        void *L2P, int dsig, void *stream) {
   const fmb_tree *t = pl->tree;
   int L = t->lmax, pL = pl->p[L];
   int nm = pL;
@@ -211,8 +214,8 @@ extern "C" int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *robin,
                        pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
                        t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],
                        t->pmin[1], t->pmin[2], (const double2 *)robin,
-                       pl->d_quad, pl->nq, pl->d_scale[L], (double *)o->d_P2M,
-                       (double *)o->d_L2P);
+                       pl->d_quad, pl->nq, pl->d_scale[L], (double *)P2M,
+                       (double *)L2P, dsig);
   } else {
     auto kern = k_bases<float>;
     if (lds > 64 * 1024)
@@ -223,10 +226,19 @@ extern "C" int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *robin,
                        pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
                        t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],
                        t->pmin[1], t->pmin[2], (const double2 *)robin,
-                       pl->d_quad, pl->nq, pl->d_scale[L], (float *)o->d_P2M,
-                       (float *)o->d_L2P);
+                       pl->d_quad, pl->nq, pl->d_scale[L], (float *)P2M,
+                       (float *)L2P, dsig);
   }
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize(st));
   return FMB_OK;
+}
+
+extern "C" int fmbh_bases(fmb_plan *pl, int op, const fmb_c64 *robin,
+                          void *stream) {
+  return launch_bases(pl, op, robin, pl->ops[op].d_P2M, pl->ops[op].d_L2P, 0, stream);
+}
+
+extern "C" int fmbh_bases_dsigma(fmb_plan *pl, int op, void *out, void *stream) {
+  return launch_bases(pl, op, NULL, out, NULL, 1, stream);
 }

@@ -150,6 +150,23 @@ __global__ void k_sweep_guess(const double *__restrict__ geo,
   x0[t] = v;
 }
 
+// dJ/dsigma_j = sum_r 2 conj(w_rj), w_rj = -x_rj (c_rj + s_rj)
+__global__ void k_adj_combine(const double2 *__restrict__ x, const double2 *__restrict__ c,
+                              const double2 *__restrict__ s, int nrhs, int N,
+                              double2 *__restrict__ g) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double gr = 0.0, gi = 0.0;
+  for (int r = 0; r < nrhs; r++) {
+    size_t i = (size_t)r * N + j;
+    double2 xv = x[i], a = make_double2(c[i].x + s[i].x, c[i].y + s[i].y);
+    double wr = -(xv.x * a.x - xv.y * a.y), wi = -(xv.x * a.y + xv.y * a.x);
+    gr += 2.0 * wr;
+    gi -= 2.0 * wi;
+  }
+  g[j] = make_double2(gr, gi);
+}
+
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -252,4 +269,13 @@ extern "C" int fmbh_sweep_guess(const fmb_plan *pl, double dk, const fmb_c64 *d_
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   return fmbh_free(d_dirs);
+}
+
+extern "C" int fmbh_adjoint_combine(const fmb_c64 *x, const fmb_c64 *c, const fmb_c64 *s,
+                                    int nrhs, int N, fmb_c64 *g, void *stream) {
+  hipLaunchKernelGGL(k_adj_combine, dim3(nblk((size_t)N)), dim3(256), 0,
+                     (hipStream_t)stream, (const double2 *)x, (const double2 *)c,
+                     (const double2 *)s, nrhs, N, (double2 *)g);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
 }

@@ -117,6 +117,21 @@ int fmbh_mmajor_table(int p, int **d_tab, void *stream);
 /* pressure at field points (field.hip): d_pts npts*3, d_dirs nrhs*3 or NULL */
 int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *d_pts,
                int npts, const double *d_dirs, fmb_c64 *out, void *stream);
+/* transposed pass pieces: (-1)^n L2P^T into L[lmax]; near^T + P2M^T into
+ * ys with the given near blocks / P2M basis (NULL: the opset's own) */
+int fmbh_l2p_t(const fmb_plan *pl, int op, int nrhs, void *stream);
+int fmbh_near_t(const fmb_plan *pl, int op, const void *near, const void *basis,
+                int nrhs, int parts, void *stream);
+/* sigma-derivative tables for the adjoint: P2M basis of dA/dsigma_j's
+ * column factor ([N][pL^2]) and its near blocks (near layout) */
+int fmbh_bases_dsigma(fmb_plan *pl, int op, void *out, void *stream);
+int fmbh_near_dsigma(fmb_plan *pl, int op, void *out, void *stream);
+/* transposed field evaluation: kt, st [nrhs][N] caller order from weights
+ * u [nrhs][npts] (field.hip) */
+int fmbh_field_t(const fmb_plan *pl, const fmb_c64 *u, int nrhs, const double *d_pts,
+                 int npts, fmb_c64 *kt, fmb_c64 *st, void *stream);
+int fmbh_adjoint_combine(const fmb_c64 *x, const fmb_c64 *c, const fmb_c64 *s,
+                         int nrhs, int N, fmb_c64 *g, void *stream);
 int fmbh_sweep_guess(const fmb_plan *pl, double dk, const fmb_c64 *d_x,
                      const double *h_dirs, int nrhs, fmb_c64 *d_x0, void *stream);
 int fmbh_rhs_plane(const fmb_plan *pl, const double *h_dirs, int nrhs,

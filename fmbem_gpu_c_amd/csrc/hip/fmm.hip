@@ -219,8 +219,10 @@ __global__ void __launch_bounds__(GS)
   }
 }
 
-// P2M: one block per leaf box
-template <typename T>
+// P2M: one block per leaf box, M[b][r][f] = sum_{e in b} B[e][f] xs[r][e].
+// FLIP multiplies coefficient f = (n, m) by (-1)^n: with B = the L2P basis
+// it is the transposed L2P of the transposed pass.
+template <typename T, bool FLIP>
 __global__ void k_p2m(const int *__restrict__ bstart,
                       const cplx<T> *__restrict__ B, const cplx<T> *__restrict__ xs,
                       int N, int ps, cplx<T> *__restrict__ M, int nrhs) {
@@ -231,6 +233,7 @@ __global__ void k_p2m(const int *__restrict__ bstart,
       cplx<T> s = cmk<T>(0, 0);
       for (int e = e0; e < e1; e++)
         cfma(s, B[(size_t)e * ps + f], xs[(size_t)r * N + e]);
+      if (FLIP && (isqrt_fix(f) & 1)) s = cmk<T>(-s.re, -s.im);
       M[((size_t)b * nrhs + r) * ps + f] = s;
     }
 }
@@ -344,15 +347,36 @@ extern "C" int fmbh_p2m(const fmb_plan *pl, int op, int nrhs, void *stream) {
   const fmb_opset *o = &pl->ops[op];
   int bs = pL * pL >= 256 ? 256 : (pL * pL >= 128 ? 128 : 64);
   if (o->prec == FMB_FP64)
-    hipLaunchKernelGGL(k_p2m<double>, dim3(t->nbox[L]), dim3(bs), 0, st,
+    hipLaunchKernelGGL((k_p2m<double, false>), dim3(t->nbox[L]), dim3(bs), 0, st,
                        t->d_box_start, (const cplx<double> *)o->d_P2M,
                        (const cplx<double> *)o->d_xs, t->nelem, pL * pL,
                        (cplx<double> *)o->d_M[L], nrhs);
   else
-    hipLaunchKernelGGL(k_p2m<float>, dim3(t->nbox[L]), dim3(bs), 0, st,
+    hipLaunchKernelGGL((k_p2m<float, false>), dim3(t->nbox[L]), dim3(bs), 0, st,
                        t->d_box_start, (const cplx<float> *)o->d_P2M,
                        (const cplx<float> *)o->d_xs, t->nelem, pL * pL,
                        (cplx<float> *)o->d_M[L], nrhs);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+// transposed L2P: L'[L] = (-1)^n L2P^T xs, the start of the transposed pass
+extern "C" int fmbh_l2p_t(const fmb_plan *pl, int op, int nrhs, void *stream) {
+  const fmb_tree *t = pl->tree;
+  int L = t->lmax, pL = pl->p[L];
+  hipStream_t st = (hipStream_t)stream;
+  const fmb_opset *o = &pl->ops[op];
+  int bs = pL * pL >= 256 ? 256 : (pL * pL >= 128 ? 128 : 64);
+  if (o->prec == FMB_FP64)
+    hipLaunchKernelGGL((k_p2m<double, true>), dim3(t->nbox[L]), dim3(bs), 0, st,
+                       t->d_box_start, (const cplx<double> *)o->d_L2P,
+                       (const cplx<double> *)o->d_xs, t->nelem, pL * pL,
+                       (cplx<double> *)o->d_L[L], nrhs);
+  else
+    hipLaunchKernelGGL((k_p2m<float, true>), dim3(t->nbox[L]), dim3(bs), 0, st,
+                       t->d_box_start, (const cplx<float> *)o->d_L2P,
+                       (const cplx<float> *)o->d_xs, t->nelem, pL * pL,
+                       (cplx<float> *)o->d_L[L], nrhs);
   FMB_LAUNCH_CHECK();
   return FMB_OK;
 }

@@ -13,7 +13,8 @@
 //   S_ii = (1/4pi) oint (e^{ik rho}-1)/(ik) dtheta
 //   T_ii = -(1/4pi) [ oint (e^{ik rho}-1)/rho dtheta + oint dtheta/rho - 2 pi i k ]
 // (K_ii = K'_ii = 0 on a flat panel).  Evaluated in FP64, stored in the
-// compute precision.
+// compute precision.  With dsig set the blocks hold dA_ij/dsigma_j instead:
+// S_ij + alpha K'_ij (i != j) and alpha/2 + S_ii.
 #include <hipcub/hipcub.hpp>
 
 #include "fmb_device.h"
@@ -101,7 +102,8 @@ __global__ void __launch_bounds__(256)
                     const double2 *__restrict__ robin,
                     const double *__restrict__ quad, int nq,
                     const double *__restrict__ quadn, int nqn, double ratio,
-                    const double *__restrict__ edge, int nedge, T *__restrict__ A) {
+                    const double *__restrict__ edge, int nedge, T *__restrict__ A,
+                    int dsig) {
   __shared__ int cpre[28], cbox[27];
   __shared__ int ncb;
   int b = blockIdx.x;
@@ -132,7 +134,10 @@ __global__ void __launch_bounds__(256)
       double2 S, Tt;
       self_terms(k, tri + 9 * (size_t)j, gj, edge, nedge, S, Tt);
       double2 aT = cm(alpha, Tt), sS = cm(sj, S), as = cm(alpha, sj);
-      v = make_double2(0.5 + 0.5 * as.x - aT.x + sS.x, 0.5 * as.y - aT.y + sS.y);
+      if (dsig)
+        v = make_double2(0.5 * alpha.x + S.x, 0.5 * alpha.y + S.y);
+      else
+        v = make_double2(0.5 + 0.5 * as.x - aT.x + sS.x, 0.5 * as.y - aT.y + sS.y);
     } else {
       double dx = gi[0] - gj[0], dy = gi[1] - gj[1], dz = gi[2] - gj[2];
       bool close = dx * dx + dy * dy + dz * dz < ratio * ratio * gj[7] * gj[7];
@@ -142,7 +147,10 @@ __global__ void __launch_bounds__(256)
       double2 aT = cm(alpha, o[3]), aKp = cm(alpha, o[2]);
       double2 t = make_double2(o[0].x + aKp.x, o[0].y + aKp.y);
       double2 st = cm(sj, t);
-      v = make_double2(-o[1].x - aT.x + st.x, -o[1].y - aT.y + st.y);
+      if (dsig)
+        v = t;
+      else
+        v = make_double2(-o[1].x - aT.x + st.x, -o[1].y - aT.y + st.y);
     }
     A[2 * (base + e)] = (T)v.x;
     A[2 * (base + e) + 1] = (T)v.y;
@@ -201,6 +209,79 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Transposed near field + transposed P2M: one block per leaf box c, one
+// wavefront per (element j of c, right-hand side) job.  Column j of A lies in
+// the blocks of the boxes b of c's neighbourhood (neighbourhoods are
+// symmetric), at the columns where b's block lists box c.  Far part:
+// sum_f B[j][f] (-1)^n M'[c][r][f], the transposed pass's coefficients being
+// kept multiplied by (-1)^n.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_near_t(const int *__restrict__ bstart, const int *__restrict__ nstart,
+             const int *__restrict__ nbox, const int *__restrict__ ncols,
+             const unsigned long long *__restrict__ boff,
+             const cplx<T> *__restrict__ A, const cplx<T> *__restrict__ B,
+             const cplx<T> *__restrict__ Mc, const cplx<T> *__restrict__ xs,
+             cplx<T> *__restrict__ ys, int N, int ps, int nrhs, int parts) {
+  __shared__ int nb_first[27], nb_row0[28], nb_col[27], nb_nc[27];
+  __shared__ unsigned long long nb_off[27];
+  __shared__ int nnb;
+  const int c = blockIdx.x;
+  const int e0 = bstart[c], nj = bstart[c + 1] - e0;
+  if (threadIdx.x == 0) {
+    int q0 = nstart[c], q1 = nstart[c + 1], acc = 0, k = 0;
+    for (int q = q0; q < q1 && k < 27; q++) {
+      int b = nbox[q], co = 0, found = 0;
+      for (int qq = nstart[b]; qq < nstart[b + 1]; qq++) {
+        int o = nbox[qq];
+        if (o == c) {
+          found = 1;
+          break;
+        }
+        co += bstart[o + 1] - bstart[o];
+      }
+      if (!found) continue; /* not reached: neighbourhoods are symmetric */
+      nb_first[k] = bstart[b];
+      nb_row0[k] = acc;
+      nb_col[k] = co;
+      nb_nc[k] = ncols[b];
+      nb_off[k] = boff[b];
+      acc += bstart[b + 1] - bstart[b];
+      k++;
+    }
+    nb_row0[k] = acc;
+    nnb = k;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int R = nb_row0[nnb];
+  for (int job = wave; job < nj * nrhs; job += (blockDim.x >> 6)) {
+    const int jj = job % nj, rh = job / nj, j = e0 + jj;
+    T sr = 0, si = 0;
+    const cplx<T> *x = xs + (size_t)rh * N;
+    for (int u = (parts & 1) ? lane : R; u < R; u += 64) {
+      int q = 0;
+      while (q + 1 < nnb && nb_row0[q + 1] <= u) q++;
+      const int r = u - nb_row0[q];
+      cplx<T> a = A[nb_off[q] + (size_t)r * nb_nc[q] + nb_col[q] + jj];
+      cplx<T> v = x[nb_first[q] + r];
+      sr += a.re * v.re - a.im * v.im;
+      si += a.re * v.im + a.im * v.re;
+    }
+    const cplx<T> *e = B + (size_t)j * ps;
+    const cplx<T> *l = Mc + ((size_t)c * nrhs + rh) * ps;
+    for (int f = (parts & 2) ? lane : ps; f < ps; f += 64) {
+      cplx<T> a = e[f], v = l[f];
+      T g = (isqrt_fix(f) & 1) ? T(-1) : T(1);
+      sr += g * (a.re * v.re - a.im * v.im);
+      si += g * (a.re * v.im + a.im * v.re);
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) ys[(size_t)rh * N + j] = cmk<T>(sr, si);
+  }
+}
+
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -247,8 +328,8 @@ extern "C" int fmbh_near_layout(fmb_plan *pl, void *stream) {
   return rc;
 }
 
-extern "C" int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *robin,
-                                  void *stream) {
+static int launch_near(fmb_plan *pl, int op, const fmb_c64 *robin, void *out,
+                       int dsig, void *stream) {
   const fmb_tree *t = pl->tree;
   hipStream_t st = (hipStream_t)stream;
   int L = t->lmax, nb = t->nbox[L];
@@ -261,7 +342,7 @@ extern "C" int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *robin,
                        pl->alpha_re, pl->alpha_im, (const double2 *)robin,
                        pl->d_quad, pl->nq, pl->d_quadn, pl->nqn,
                        pl->prm.near_ratio, pl->d_edge, pl->prm.nedge,
-                       (double *)o->d_near);
+                       (double *)out, dsig);
   else
     hipLaunchKernelGGL(k_near_assemble<float>, dim3(nb), dim3(256), 0, st,
                        t->d_box_start, t->d_near_start, t->d_near_box,
@@ -269,10 +350,19 @@ extern "C" int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *robin,
                        pl->alpha_re, pl->alpha_im, (const double2 *)robin,
                        pl->d_quad, pl->nq, pl->d_quadn, pl->nqn,
                        pl->prm.near_ratio, pl->d_edge, pl->prm.nedge,
-                       (float *)o->d_near);
+                       (float *)out, dsig);
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize(st));
   return FMB_OK;
+}
+
+extern "C" int fmbh_near_assemble(fmb_plan *pl, int op, const fmb_c64 *robin,
+                                  void *stream) {
+  return launch_near(pl, op, robin, pl->ops[op].d_near, 0, stream);
+}
+
+extern "C" int fmbh_near_dsigma(fmb_plan *pl, int op, void *out, void *stream) {
+  return launch_near(pl, op, NULL, out, 1, stream);
 }
 
 extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
@@ -297,6 +387,33 @@ extern "C" int fmbh_l2p_near(const fmb_plan *pl, int op, int nrhs, int parts,
                        pl->d_near_ncol, off, (const cplx<float> *)o->d_near,
                        (const cplx<float> *)o->d_L2P,
                        (const cplx<float> *)o->d_L[L],
+                       (const cplx<float> *)o->d_xs, (cplx<float> *)o->d_ys,
+                       t->nelem, pL * pL, nrhs, parts);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+extern "C" int fmbh_near_t(const fmb_plan *pl, int op, const void *near,
+                           const void *basis, int nrhs, int parts, void *stream) {
+  const fmb_tree *t = pl->tree;
+  hipStream_t st = (hipStream_t)stream;
+  int L = t->lmax, nb = t->nbox[L], pL = pl->p[L];
+  const fmb_opset *o = &pl->ops[op];
+  const void *A = near ? near : o->d_near;
+  const void *B = basis ? basis : o->d_P2M;
+  const unsigned long long *off = (const unsigned long long *)pl->d_near_off;
+  if (o->prec == FMB_FP64)
+    hipLaunchKernelGGL(k_near_t<double>, dim3(nb), dim3(256), 0, st,
+                       t->d_box_start, t->d_near_start, t->d_near_box,
+                       pl->d_near_ncol, off, (const cplx<double> *)A,
+                       (const cplx<double> *)B, (const cplx<double> *)o->d_M[L],
+                       (const cplx<double> *)o->d_xs, (cplx<double> *)o->d_ys,
+                       t->nelem, pL * pL, nrhs, parts);
+  else
+    hipLaunchKernelGGL(k_near_t<float>, dim3(nb), dim3(256), 0, st,
+                       t->d_box_start, t->d_near_start, t->d_near_box,
+                       pl->d_near_ncol, off, (const cplx<float> *)A,
+                       (const cplx<float> *)B, (const cplx<float> *)o->d_M[L],
                        (const cplx<float> *)o->d_xs, (cplx<float> *)o->d_ys,
                        t->nelem, pL * pL, nrhs, parts);
   FMB_LAUNCH_CHECK();

@@ -15,6 +15,7 @@
 
 #include "../fmb_internal.h"
 #include "../hip/fmb_hip.h"
+#include "host.h"
 
 typedef double _Complex zc;
 
@@ -30,12 +31,22 @@ static fmb_c64 fromz(zc z) {
   return v;
 }
 
-int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
-              int restart, double tol, int maxit, fmb_gmres_info *info,
-              void *stream) {
+/* y = A x, or A^T x for the transposed system */
+static int apply(fmb_plan *pl, int trans, int prec, const fmb_c64 *x, fmb_c64 *y,
+                 int nrhs, void *stream) {
+  if (!trans) return fmb_matvec(pl, prec, x, y, nrhs, stream);
+  int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
+  return fmbx_matvec_t(pl, op, 3, NULL, NULL, x, y, nrhs, stream);
+}
+
+static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
+                     int nrhs, int restart, double tol, int maxit,
+                     fmb_gmres_info *info, void *stream) {
   if (!pl || !pl->ready || !b || !x || nrhs < 1 || nrhs > pl->nrhs_max || restart < 1 ||
       !(tol > 0) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
+  if (trans && pl->prm.rcr_path == FMB_RCR_VALU)
+    return fmb_set_error(FMB_ERR_ARG, "transposed solve needs rcr_path FMB_RCR_AUTO");
   double t0 = now_s();
   const int N = pl->tree->nelem, m = restart;
   const int work = pl->work_prec;
@@ -73,7 +84,7 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
     if (bn[r] == 0.0) bn[r] = 1.0;
   while (1) {
     /* true residual r = b - A x in FP64 */
-    if ((rc = fmb_matvec(pl, FMB_FP64, x, dw, nrhs, stream))) goto done;
+    if ((rc = apply(pl, trans, FMB_FP64, x, dw, nrhs, stream))) goto done;
     mv64++;
     {
       fmb_c64 *one = hh, *mone = hh2;
@@ -111,7 +122,7 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
     int j;
     for (j = 0; j < m && iters < maxit; j++) {
       fmb_c64 *Vj = dV + (size_t)j * N * nrhs;
-      if ((rc = fmb_matvec(pl, work, Vj, dw, nrhs, stream))) goto done;
+      if ((rc = apply(pl, trans, work, Vj, dw, nrhs, stream))) goto done;
       if (work == FMB_MIXED)
         mvw++;
       else
@@ -211,4 +222,16 @@ done:
   free(hn);
   free(active);
   return rc;
+}
+
+int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
+              int restart, double tol, int maxit, fmb_gmres_info *info,
+              void *stream) {
+  return gmres_run(pl, 0, b, x, nrhs, restart, tol, maxit, info, stream);
+}
+
+int fmb_gmres_transpose(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
+                        int restart, double tol, int maxit, fmb_gmres_info *info,
+                        void *stream) {
+  return gmres_run(pl, 1, b, x, nrhs, restart, tol, maxit, info, stream);
 }

@@ -13,4 +13,11 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream);
 void fmbx_rcr_free(fmb_plan *pl);
 int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
                  void *stream);
+/* transposed translations, coefficients kept multiplied by (-1)^n */
+int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
+                   void *stream);
+/* transposed FMM pass (plan.c); near / basis NULL: the opset's own */
+int fmbx_matvec_t(fmb_plan *pl, int op, int parts, const void *near,
+                  const void *basis, const fmb_c64 *x, fmb_c64 *y, int nrhs,
+                  void *stream);
 #endif

@@ -141,7 +141,8 @@ static void opset_free_rot(fmb_opset *o) {
   o->d_H = o->d_Hpad = NULL;
   for (int i = 0; i < 3; i++) {
     fmbh_free(o->d_phase[i]);
-    o->d_phase[i] = NULL;
+    fmbh_free(o->d_phase_c[i]);
+    o->d_phase[i] = o->d_phase_c[i] = NULL;
   }
   o->hpad_stride = 0;
 }
@@ -607,6 +608,47 @@ out:
   if (tm)
     for (int i = 0; i < 4 + 2 * FMB_MAXLEV; i++) fmbh_event_destroy(ev[i]);
   return rc;
+}
+
+/* y = A^T x: L2P^T, L2L^T upward, M2L^T and M2M^T downward, near^T + P2M^T
+ * (the forward pass run backwards with every operator transposed; the
+ * coefficients carry a (-1)^n factor, see fmbx_rcr_run_t).  near / basis
+ * replace the opset's near blocks and P2M basis when not NULL. */
+int fmbx_matvec_t(fmb_plan *pl, int op, int parts, const void *near,
+                  const void *basis, const fmb_c64 *x, fmb_c64 *y, int nrhs,
+                  void *stream) {
+  const fmb_tree *t = pl->tree;
+  fmb_opset *o = &pl->ops[op];
+  int rc, L = t->lmax;
+  if ((rc = fmbh_gather(pl, op, x, nrhs, stream))) return rc;
+  if (parts & 2) {
+    for (int l = 2; l <= L; l++) {
+      size_t bytes = (size_t)t->nbox[l] * nrhs * pl->p[l] * pl->p[l] * o->csz;
+      if (l < L && (rc = fmbh_memset(o->d_L[l], 0, bytes, stream))) return rc;
+      if ((rc = fmbh_memset(o->d_M[l], 0, bytes, stream))) return rc;
+    }
+    if ((rc = fmbh_l2p_t(pl, op, nrhs, stream))) return rc;
+    for (int l = L; l > 2; l--)
+      if ((rc = fmbx_rcr_run_t(pl, op, 2, l, nrhs, stream))) return rc;
+    for (int l = 2; l <= L; l++) {
+      if (l > 2 && (rc = fmbx_rcr_run_t(pl, op, 1, l, nrhs, stream))) return rc;
+      if ((rc = fmbx_rcr_run_t(pl, op, 0, l, nrhs, stream))) return rc;
+    }
+  }
+  if ((rc = fmbh_near_t(pl, op, near, basis, nrhs, parts, stream))) return rc;
+  return fmbh_scatter(pl, op, y, nrhs, stream);
+}
+
+int fmb_matvec_transpose(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
+                         fmb_c64 *y, int nrhs, void *stream) {
+  if (!pl || !pl->ready || parts < 1 || parts > 3 || !x || !y || nrhs < 1 ||
+      nrhs > pl->nrhs_max)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_matvec_transpose: invalid arguments");
+  if (pl->prm.rcr_path == FMB_RCR_VALU)
+    return fmb_set_error(FMB_ERR_ARG,
+                         "fmb_matvec_transpose: needs rcr_path FMB_RCR_AUTO");
+  int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
+  return fmbx_matvec_t(pl, op, parts, NULL, NULL, x, y, nrhs, stream);
 }
 
 int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,

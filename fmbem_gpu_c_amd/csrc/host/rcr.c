@@ -258,20 +258,26 @@ int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
         ph[2 * ((size_t)b * pl->Pmax + m) + 1] = sin(m * phi);
       }
     }
-    for (int op = 0; op < pl->nops && !rc; op++) {
-      fmb_opset *o = &pl->ops[op];
-      size_t n = (size_t)2 * nph[q] * pl->Pmax;
-      rc = fmbh_malloc(&o->d_phase[q], o->csz / 2 * n);
-      if (rc) break;
-      if (o->prec == FMB_FP64) {
-        rc = fmbh_h2d(o->d_phase[q], ph, sizeof(double) * n, stream);
-        if (!rc) rc = fmbh_sync(stream);
-      } else {
-        float *f = (float *)malloc(sizeof(float) * n);
-        for (size_t i = 0; i < n; i++) f[i] = (float)ph[i];
-        rc = fmbh_h2d(o->d_phase[q], f, sizeof(float) * n, stream);
-        if (!rc) rc = fmbh_sync(stream);
-        free(f);
+    /* cj = 0: e^{i m phi}; cj = 1: e^{-i m phi} for the transposed pass */
+    size_t n = (size_t)2 * nph[q] * pl->Pmax;
+    for (int cj = 0; cj < 2 && !rc; cj++) {
+      if (cj)
+        for (size_t i = 1; i < n; i += 2) ph[i] = -ph[i];
+      for (int op = 0; op < pl->nops && !rc; op++) {
+        fmb_opset *o = &pl->ops[op];
+        void **dst = cj ? &o->d_phase_c[q] : &o->d_phase[q];
+        rc = fmbh_malloc(dst, o->csz / 2 * n);
+        if (rc) break;
+        if (o->prec == FMB_FP64) {
+          rc = fmbh_h2d(*dst, ph, sizeof(double) * n, stream);
+          if (!rc) rc = fmbh_sync(stream);
+        } else {
+          float *f = (float *)malloc(sizeof(float) * n);
+          for (size_t i = 0; i < n; i++) f[i] = (float)ph[i];
+          rc = fmbh_h2d(*dst, f, sizeof(float) * n, stream);
+          if (!rc) rc = fmbh_sync(stream);
+          free(f);
+        }
       }
     }
     free(ph);
@@ -421,6 +427,69 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
     j.src = pairs->d_par;
     j.in = o->d_L[l - 1];
     j.out = o->d_L[l];
+  }
+  return fmbh_rcr(&j, o->prec, nrhs, stream);
+}
+
+/* The transposed translations.  The rotation blocks H_n(beta) are symmetric
+ * and every coaxial |m| block satisfies C^T = P C P with P = diag((-1)^n),
+ * so the transpose of T = Pout H C H Pin (Pin = e^{i m phi},
+ * Pout = e^{-i m' phi}) is P (Pin H C' H Pout) P, where C' is C for M2L
+ * and, for the scaled M2M / L2L blocks, the other direction's block:
+ *   (diag(1/w_{l-1}) RR diag(w_l))^T = P diag(w_l) RR diag(1/w_{l-1}) P.
+ * The transposed pass keeps its coefficients multiplied by P, so each
+ * transposed translation is the forward kernel with conjugated phases:
+ *   kind 0  M2L^T at l: L' (l) -> M' (l), pairs with source and target swapped;
+ *   kind 1  M2M^T l-1 -> l: the L2L schedule and blocks, M2M's octant angles;
+ *   kind 2  L2L^T l -> l-1: the M2M schedule and blocks, L2L's octant angles. */
+int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
+                   void *stream) {
+  const fmb_rcr_tab *tb = &pl->rcr[kind == 0 ? 0 : (kind == 1 ? 2 : 1)][l];
+  if (!tb->mode[op])
+    return fmb_set_error(FMB_ERR_ARG,
+                         "transposed translations need the matrix-core kernels");
+  const fmb_tree *t = pl->tree;
+  const fmb_opset *o = &pl->ops[op];
+  const fmb_rcr_tab *pairs = kind == 0 ? tb : &pl->rcr[1][l];
+  fmbh_rcr_job j;
+  memset(&j, 0, sizeof(j));
+  j.mode = tb->mode[op];
+  j.nt = tb->nt[op];
+  j.mt = tb->mt[op];
+  j.chunks = pairs->d_chunks;
+  j.nchunks = pairs->nchunks;
+  j.chunk_pairs = RCR_CHUNK;
+  j.H = o->d_Hpad;
+  j.hstride = o->hpad_stride;
+  j.C = tb->d_Cpad[op];
+  j.cstride = tb->cpad_stride;
+  j.P = tb->P;
+  j.pin = tb->pin;
+  j.pout = tb->pout;
+  j.sched = tb->d_sched[op];
+  j.sched_len = tb->sched_len[op];
+  j.gws = o->d_gws;
+  j.gws_grid = o->gws_grid;
+  j.phase = o->d_phase_c[kind];
+  j.pstride = pl->Pmax;
+  if (kind == 0) {
+    j.info = pl->d_off_info;
+    j.tgt = t->d_pair_src[l];
+    j.src = t->d_pair_tgt[l];
+    j.in = o->d_L[l];
+    j.out = o->d_M[l];
+  } else if (kind == 1) {
+    j.info = pl->d_info_oct[0];
+    j.tgt = pairs->d_chi;
+    j.src = pairs->d_par;
+    j.in = o->d_M[l - 1];
+    j.out = o->d_M[l];
+  } else {
+    j.info = pl->d_info_oct[1];
+    j.tgt = pairs->d_par;
+    j.src = pairs->d_chi;
+    j.in = o->d_L[l];
+    j.out = o->d_L[l - 1];
   }
   return fmbh_rcr(&j, o->prec, nrhs, stream);
 }

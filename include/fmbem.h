@@ -234,6 +234,36 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
  * alias.  The reference has no solver. */
 int fmb_sweep_guess(const fmb_plan *pl, double k_from, const fmb_c64 *x_from,
                     const double *dirs, int nrhs, fmb_c64 *x0, void *stream);
+/* y = A^T x (plain transpose, no conjugation) of the operator fmb_matvec
+ * applies: the FMM pass run backwards with every operator transposed (L2P^T,
+ * L2L^T, M2L^T, M2M^T, P2M^T; near-field blocks read by column).  parts as
+ * for fmb_matvec_parts.  Needs rcr_path FMB_RCR_AUTO.  The reference has no
+ * transposed operator; it serves adjoint solves. */
+int fmb_matvec_transpose(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
+                         fmb_c64 *y, int nrhs, void *stream);
+/* fmb_gmres on A^T x = b (same options, refinement and info). */
+int fmb_gmres_transpose(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
+                        int restart, double tol, int maxit, fmb_gmres_info *info,
+                        void *stream);
+
+/* Adjoint gradient (the reference has none).  For the field-point objective
+ *   J = sum_r sum_q |p_r(y_q)|^2,
+ * with p_r the field of fmb_field_eval from the surface solution x ([nrhs][N]
+ * device, A x = b solved beforehand; dirs: nrhs*3 host doubles of the
+ * incident plane waves, or NULL for the scattered field) at pts (npts*3 host
+ * doubles), returns J (host) and dJ_dsigma[j] = dJ/dRe sigma_j +
+ * i dJ/dIm sigma_j ([N] device, caller order) for the plan's Robin
+ * coefficients sigma (dp/dn = sigma p).  One adjoint solve A^T lambda =
+ * F^T conj(p) (fmb_gmres_transpose with restart/tol/maxit; its report in
+ * info, may be NULL), then dJ/dsigma_j = 2 conj(-x_j ((C^T lambda)_j +
+ * (Sf^T conj(p))_j)) with C = dA/dsigma's column factor S + alpha K' +
+ * alpha I/2 and Sf the single layer at the field points.  For an admittance
+ * Y with sigma = i k Y: dJ/dY = -i k dJ/dsigma.  Needs rcr_path
+ * FMB_RCR_AUTO. */
+int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dirs,
+                     const double *pts, int npts, int restart, double tol, int maxit,
+                     double *J, fmb_c64 *dJ_dsigma, fmb_gmres_info *info,
+                     void *stream);
 /* Precision of the Arnoldi matvecs of fmb_gmres on this plan (FMB_MIXED only
  * on a plan created with FMB_MIXED).  The reference has no solver. */
 int fmb_plan_set_work_precision(fmb_plan *pl, int prec);
