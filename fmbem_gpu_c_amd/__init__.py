@@ -110,7 +110,8 @@ _SIGS = {
                                            ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "fmb_adjoint_grad": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp,
-                                        _vp, _vp, _vp]),
+                                        _vp, _vp, _vp, _vp, _vp]),
+    "fmb_tree_vertex_normals": (ctypes.c_int, [_vp, _vp]),
     "fmb_plan_wavenumber": (ctypes.c_int, [_vp, _vp]),
     "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -147,11 +148,12 @@ def _ptr(a, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
-def _dev(t):
+def _dev(t, dtype=None):
     import torch
-    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128
+    dtype = torch.complex128 if dtype is None else dtype
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype
             and t.is_contiguous()):
-        raise FmbError(-1, "expected a contiguous complex128 CUDA tensor")
+        raise FmbError(-1, "expected a contiguous %s CUDA tensor" % str(dtype).split(".")[-1])
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -239,6 +241,13 @@ class Tree:
         num = np.zeros(len(self.level(l)), dtype=np.int32)
         _check(lib().fmb_tree_num_sr(self.h, l, _ptr(num, ctypes.c_int)))
         return num
+
+    def vertex_normals(self):
+        """Unit area-weighted vertex normals (nvert x 3)."""
+        nv = self.mesh.arrays()[0].shape[0]
+        out = np.zeros((nv, 3), dtype=np.float64)
+        _check(lib().fmb_tree_vertex_normals(self.h, _ptr(out, ctypes.c_double)))
+        return out
 
     def elem_box(self):
         n = self.mesh.arrays()[1].shape[0]
@@ -404,11 +413,14 @@ class Plan:
             _check(rc)
         return x, info
 
-    def adjoint_grad(self, x, pts, dirs=None, restart=30, tol=1e-10, maxit=1000):
+    def adjoint_grad(self, x, pts, dirs=None, restart=30, tol=1e-10, maxit=1000,
+                     vertices=False):
         """J = sum |p|^2 at pts (M x 3) for the surface solution x and its
         gradient dJ/dRe sigma + i dJ/dIm sigma per element (complex128 CUDA
         tensor, caller order) with respect to the Robin coefficients
-        (fmb_adjoint_grad).  Returns (J, grad, info of the adjoint solve)."""
+        (fmb_adjoint_grad).  Returns (J, grad, info of the adjoint solve), and
+        with vertices=True also dJ/dV (nvert x 3) and dJ/d(vertex normal
+        displacement) (nvert) as float64 CUDA tensors."""
         import torch
         nrhs = 1 if x.dim() == 1 else x.shape[0]
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
@@ -419,13 +431,22 @@ class Plan:
                 raise FmbError(-1, "one incident direction per right-hand side")
         n = x.shape[-1]
         g = torch.empty(n, dtype=torch.complex128, device=x.device)
+        dV = dvn = None
+        if vertices:
+            nv = self.tree.mesh.arrays()[0].shape[0]
+            dV = torch.empty((nv, 3), dtype=torch.float64, device=x.device)
+            dvn = torch.empty(nv, dtype=torch.float64, device=x.device)
         J = ctypes.c_double()
         info = GmresInfo()
         _check(lib().fmb_adjoint_grad(self.h, _dev(x), nrhs,
                                       ctypes.c_void_p(d.ctypes.data) if d is not None else None,
                                       ctypes.c_void_p(pts.ctypes.data), pts.shape[0], restart,
-                                      tol, maxit, ctypes.byref(J), _dev(g), ctypes.byref(info),
-                                      _stream()))
+                                      tol, maxit, ctypes.byref(J), _dev(g),
+                                      _dev(dV, torch.float64) if vertices else None,
+                                      _dev(dvn, torch.float64) if vertices else None,
+                                      ctypes.byref(info), _stream()))
+        if vertices:
+            return J.value, g, info, dV, dvn
         return J.value, g, info
 
     def set_work_precision(self, prec):

@@ -39,6 +39,9 @@ struct fmb_tree {
   int *d_elem_box;           /* sorted position -> leaf box                */
   double *d_tri;             /* sorted order: 9 doubles per element        */
   double *d_geo;             /* sorted order: centroid(3) normal(3) area radius */
+  int *d_fv;                 /* sorted order: the element's 3 vertex ids    */
+  int nvert;
+  double *d_vnorm;           /* [nvert][3] unit area-weighted vertex normals */
   int npairs[FMB_MAXLEV];
   int *d_pair_tgt[FMB_MAXLEV], *d_pair_src[FMB_MAXLEV];
   int *h_off_start[FMB_MAXLEV]; /* host: FMB_NOFF+1 bucket ranges           */

@@ -16,91 +16,6 @@ namespace {
 
 constexpr int WG = 64;
 
-__device__ __forceinline__ int tri_idx(int n, int m) { return n * (n + 1) / 2 + m; }
-
-// Fill Rt[(n,m)] = R_n^m(v), n <= nm, in LDS.  Scratch in LDS as given.
-__device__ void r_table(int nm, double k, const double *v, double *jn,
-                        double *diag, double *Pt, double2 *ph, double2 *Rt,
-                        double *geo) {
-  int lane = threadIdx.x;
-  if (lane == 0) {
-    double rho2 = v[0] * v[0] + v[1] * v[1];
-    double r = sqrt(rho2 + v[2] * v[2]);
-    double ct = 1.0, st = 0.0, phi = 0.0;
-    if (r > 0) {
-      ct = v[2] / r;
-      st = sqrt(rho2) / r;
-      phi = atan2(v[1], v[0]);
-    }
-    geo[0] = ct;
-    geo[1] = st;
-    geo[2] = phi;
-    dev_sph_jn(nm, k * r, jn);
-    double d = 1.0 / sqrt(4.0 * M_PI);
-    diag[0] = d;
-    for (int m = 1; m <= nm; m++) {
-      d *= sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
-      diag[m] = d;
-    }
-  }
-  __syncthreads();
-  double ct = geo[0], phi = geo[2];
-  for (int m = lane; m <= nm; m += WG) {
-    double p2 = 0.0, p1 = diag[m];
-    Pt[tri_idx(m, m)] = p1;
-    for (int n = m + 1; n <= nm; n++) {
-      double pv;
-      if (n == m + 1)
-        pv = sqrt(2.0 * m + 3.0) * ct * p1;
-      else {
-        double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
-        double b = sqrt((((double)(n - 1) * (n - 1)) - (double)m * m) /
-                        (4.0 * (n - 1) * (n - 1) - 1.0));
-        pv = a * (ct * p1 - b * p2);
-      }
-      Pt[tri_idx(n, m)] = pv;
-      p2 = p1;
-      p1 = pv;
-    }
-    double s, c;
-    sincos(m * phi, &s, &c);
-    ph[m] = make_double2(c, s);
-  }
-  __syncthreads();
-  for (int f = lane; f < (nm + 1) * (nm + 1); f += WG) {
-    int n = (int)sqrt((double)f);
-    while (n * n > f) n--;
-    while ((n + 1) * (n + 1) <= f) n++;
-    int m = f - n * n - n, am = m < 0 ? -m : m;
-    double a = jn[n] * Pt[tri_idx(n, am)];
-    double2 e = ph[am];
-    Rt[f] = make_double2(a * e.x, m < 0 ? -a * e.y : a * e.y);
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ double2 rget(const double2 *Rt, int n, int m) {
-  if (n < 0 || (m < 0 ? -m : m) > n) return make_double2(0.0, 0.0);
-  return Rt[n * n + n + m];
-}
-
-// dR_n^m/dn (integral.cu:284-331)
-__device__ __forceinline__ double2 dRdn(const double2 *Rt, int n, int m,
-                                        double k, const double *nr) {
-  double2 a1 = rget(Rt, n + 1, m + 1), a2 = rget(Rt, n - 1, m + 1);
-  double2 b1 = rget(Rt, n + 1, m - 1), b2 = rget(Rt, n - 1, m - 1);
-  double2 c1 = rget(Rt, n - 1, m), c2 = rget(Rt, n + 1, m);
-  double f0 = bcoef(n + 1, -m - 1), f1 = bcoef(n, m);
-  double f2 = bcoef(n + 1, m - 1), f3 = bcoef(n, -m);
-  double f4 = acoef(n - 1, m), f5 = acoef(n, m);
-  double t0r = f0 * a1.x - f1 * a2.x, t0i = f0 * a1.y - f1 * a2.y;
-  double t1r = f2 * b1.x - f3 * b2.x, t1i = f2 * b1.y - f3 * b2.y;
-  double t2r = f4 * c1.x - f5 * c2.x, t2i = f4 * c1.y - f5 * c2.y;
-  double hx = 0.5 * k * nr[0], hy = 0.5 * k * nr[1], kz = k * nr[2];
-  return make_double2(hx * t0r + hy * t0i + hx * t1r - hy * t1i + kz * t2r,
-                      hx * t0i - hy * t0r + hx * t1i + hy * t1r + kz * t2i);
-}
-
 template <typename T>
 __global__ void __launch_bounds__(WG)
     k_bases(int N, int pL, double k, double alre, double alim,

@@ -132,6 +132,10 @@ int fmbh_field_t(const fmb_plan *pl, const fmb_c64 *u, int nrhs, const double *d
                  int npts, fmb_c64 *kt, fmb_c64 *st, void *stream);
 int fmbh_adjoint_combine(const fmb_c64 *x, const fmb_c64 *c, const fmb_c64 *s,
                          int nrhs, int N, fmb_c64 *g, void *stream);
+/* vertex gradients of the field objective (shape.hip) */
+int fmbh_shape_grad(const fmb_plan *pl, const fmb_c64 *x, const fmb_c64 *lam, int nrhs,
+                    const void *Lx, const void *Mt, const fmb_c64 *u, const double *d_pts,
+                    int npts, const double *d_dirs, double *dV, double *dvn, void *stream);
 int fmbh_sweep_guess(const fmb_plan *pl, double dk, const fmb_c64 *d_x,
                      const double *h_dirs, int nrhs, fmb_c64 *d_x0, void *stream);
 int fmbh_rhs_plane(const fmb_plan *pl, const double *h_dirs, int nrhs,

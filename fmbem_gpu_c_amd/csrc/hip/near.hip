@@ -35,59 +35,6 @@ __global__ void k_near_sizes(const int *bstart, const int *nstart,
   bsz[b] = (unsigned long long)(bstart[b + 1] - bstart[b]) * c;
 }
 
-__device__ void self_terms(double k, const double *tri, const double *c,
-                           const double *edge, int nedge, double2 &S,
-                           double2 &T) {
-  double sSr = 0, sSi = 0, sTr = 0, sTi = 0, sinv = 0;
-  for (int e = 0; e < 3; e++) {
-    const double *P = tri + 3 * e, *Q = tri + 3 * ((e + 1) % 3);
-    double ev[3], L = 0;
-    for (int i = 0; i < 3; i++) {
-      ev[i] = Q[i] - P[i];
-      L += ev[i] * ev[i];
-    }
-    L = sqrt(L);
-    double tp = 0;
-    for (int i = 0; i < 3; i++) {
-      ev[i] /= L;
-      tp += (c[i] - P[i]) * ev[i];
-    }
-    double h = 0;
-    for (int i = 0; i < 3; i++) {
-      double f = P[i] + tp * ev[i] - c[i];
-      h += f * f;
-    }
-    h = sqrt(h);
-    double thP = atan2(-tp, h), thQ = atan2(L - tp, h);
-    sinv += (sin(thQ) - sin(thP)) / h;
-    double half = 0.5 * (thQ - thP), mid = 0.5 * (thQ + thP);
-    for (int q = 0; q < nedge; q++) {
-      double th = mid + half * edge[2 * q];
-      double rho = h / cos(th);
-      double w = half * edge[2 * q + 1];
-      double s2, c2;
-      sincos(0.5 * k * rho, &s2, &c2);
-      // gs = 2 sin(k rho/2) e^{i k rho/2} / k
-      double gr, gi;
-      if (k == 0.0) {
-        gr = rho;
-        gi = 0;
-      } else {
-        gr = 2.0 * s2 * c2 / k;
-        gi = 2.0 * s2 * s2 / k;
-      }
-      sSr += w * gr;
-      sSi += w * gi;
-      // i k gs / rho
-      sTr += w * (-k * gi / rho);
-      sTi += w * (k * gr / rho);
-    }
-  }
-  const double f = 1.0 / (4.0 * M_PI);
-  S = make_double2(f * sSr, f * sSi);
-  T = make_double2(-f * (sTr + sinv), -f * (sTi - 2.0 * M_PI * k));
-}
-
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

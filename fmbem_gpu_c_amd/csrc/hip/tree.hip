@@ -234,12 +234,13 @@ __global__ void k_near_fill(const int *keys, int nb, int l, const int *start,
 }
 
 __global__ void k_sorted_geo(const double *V, const int *F, const int *perm,
-                             int n, double *tri, double *geo, int *iperm) {
+                             int n, double *tri, double *geo, int *iperm, int *fv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int e = perm[i];
   iperm[e] = i;
   double P[3][3];
+  for (int v = 0; v < 3; v++) fv[3 * i + v] = F[3 * e + v];
   for (int v = 0; v < 3; v++)
     for (int q = 0; q < 3; q++) {
       P[v][q] = V[3 * F[3 * e + v] + q];
@@ -540,9 +541,10 @@ extern "C" int fmbh_sorted_geometry(fmb_tree *t, const double *d_V,
   int rc = fmbh_malloc((void **)&t->d_tri, sizeof(double) * 9 * n);
   if (!rc) rc = fmbh_malloc((void **)&t->d_geo, sizeof(double) * 8 * n);
   if (!rc) rc = fmbh_malloc((void **)&t->d_iperm, sizeof(int) * n);
+  if (!rc) rc = fmbh_malloc((void **)&t->d_fv, sizeof(int) * 3 * n);
   if (rc) return rc;
   hipLaunchKernelGGL(k_sorted_geo, dim3(nblk(n)), dim3(256), 0, st, d_V, d_F,
-                     t->d_perm, n, t->d_tri, t->d_geo, t->d_iperm);
+                     t->d_perm, n, t->d_tri, t->d_geo, t->d_iperm, t->d_fv);
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize(st));
   return FMB_OK;

@@ -14,7 +14,12 @@
  * 2 conj(w_j), summed over right-hand sides.
  *
  * A^T products run the transposed FMM pass (plan.c, rcr.c); C^T lambda runs
- * the same pass with the sigma-derivative P2M basis and near blocks. */
+ * the same pass with the sigma-derivative P2M basis and near blocks.
+ *
+ * Vertex gradients (shape.hip): dJ = 2 Re [u^T dF x + lambda^T (db - dA x)]
+ * split into per-element terms, with the forward pass's leaf local
+ * expansions of x and the transposed pass's leaf multipole expansions of
+ * lambda standing in for the far rows and columns of A. */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,11 +30,15 @@
 
 int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dirs,
                      const double *pts, int npts, int restart, double tol, int maxit,
-                     double *J, fmb_c64 *dJ_dsigma, fmb_gmres_info *info,
-                     void *stream) {
+                     double *J, fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
+                     fmb_gmres_info *info, void *stream) {
+  const int shape = dJ_dV || dJ_dvn;
   if (!pl || !pl->ready || !x || nrhs < 1 || nrhs > pl->nrhs_max || !pts ||
-      npts < 1 || !dJ_dsigma || restart < 1 || !(tol > 0) || maxit < 1)
+      npts < 1 || !(dJ_dsigma || shape) || restart < 1 || !(tol > 0) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_adjoint_grad: invalid arguments");
+  if (shape && !dirs)
+    return fmb_set_error(FMB_ERR_ARG,
+                         "fmb_adjoint_grad: vertex gradients need the incident directions");
   if (pl->prm.rcr_path == FMB_RCR_VALU)
     return fmb_set_error(FMB_ERR_ARG, "fmb_adjoint_grad: needs rcr_path FMB_RCR_AUTO");
   const fmb_tree *t = pl->tree;
@@ -38,7 +47,9 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
   double *d_pts = NULL, *d_dirs = NULL;
   fmb_c64 *d_p = NULL, *d_kt = NULL, *d_st = NULL, *d_lam = NULL, *d_c = NULL;
   fmb_c64 *h_p = NULL;
-  void *d_bs = NULL, *d_nc = NULL;
+  void *d_bs = NULL, *d_nc = NULL, *d_lx = NULL;
+  double *d_dv = NULL;
+  const size_t lbytes = (size_t)t->nbox[L] * nrhs * pL * pL * sizeof(fmb_c64);
   int rc = fmbh_malloc((void **)&d_pts, sizeof(double) * 3 * (size_t)npts);
   if (!rc) rc = fmbh_h2d(d_pts, pts, sizeof(double) * 3 * (size_t)npts, stream);
   if (!rc && dirs) rc = fmbh_malloc((void **)&d_dirs, sizeof(double) * 3 * nrhs);
@@ -62,6 +73,12 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
     if (J) *J = s;
   }
   if ((rc = fmbh_h2d(d_p, h_p, sizeof(fmb_c64) * np, stream))) goto done;
+  /* leaf local expansions of x from an FP64 pass (vertex gradients) */
+  if (shape) {
+    if ((rc = fmbh_malloc(&d_lx, lbytes))) goto done;
+    if ((rc = fmb_matvec(pl, FMB_FP64, x, d_c, nrhs, stream))) goto done;
+    if ((rc = fmbh_d2d(d_lx, pl->ops[0].d_L[L], lbytes, stream))) goto done;
+  }
   /* F^T u and Sf^T u */
   if ((rc = fmbh_field_t(pl, d_p, nrhs, d_pts, npts, d_kt, d_st, stream))) goto done;
   /* A^T lambda = F^T u */
@@ -76,8 +93,22 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
   if ((rc = fmbh_bases_dsigma(pl, 0, d_bs, stream))) goto done;
   if ((rc = fmbh_near_dsigma(pl, 0, d_nc, stream))) goto done;
   if ((rc = fmbx_matvec_t(pl, 0, 3, d_nc, d_bs, d_lam, d_c, nrhs, stream))) goto done;
-  rc = fmbh_adjoint_combine(x, d_c, d_st, nrhs, N, dJ_dsigma, stream);
-  if (!rc) rc = fmbh_sync(stream);
+  if (dJ_dsigma && (rc = fmbh_adjoint_combine(x, d_c, d_st, nrhs, N, dJ_dsigma, stream)))
+    goto done;
+  /* vertex gradients: the last FP64 transposed pass left lambda's leaf
+   * multipole expansions in the FP64 opset */
+  if (shape) {
+    double *dv = dJ_dV;
+    if (!dv) {
+      if ((rc = fmbh_malloc((void **)&d_dv, sizeof(double) * 3 * (size_t)t->nvert)))
+        goto done;
+      dv = d_dv;
+    }
+    if ((rc = fmbh_shape_grad(pl, x, d_lam, nrhs, d_lx, pl->ops[0].d_M[L], d_p, d_pts, npts,
+                              d_dirs, dv, dJ_dvn, stream)))
+      goto done;
+  }
+  rc = fmbh_sync(stream);
 done:
   free(h_p);
   fmbh_free(d_pts);
@@ -89,5 +120,7 @@ done:
   fmbh_free(d_c);
   fmbh_free(d_bs);
   fmbh_free(d_nc);
+  fmbh_free(d_lx);
+  fmbh_free(d_dv);
   return rc;
 }

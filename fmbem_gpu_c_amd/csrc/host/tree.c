@@ -48,6 +48,8 @@ void fmb_tree_destroy(fmb_tree *t) {
   fmbh_free(t->d_elem_box);
   fmbh_free(t->d_tri);
   fmbh_free(t->d_geo);
+  fmbh_free(t->d_fv);
+  fmbh_free(t->d_vnorm);
   fmbh_free(t->d_near_start);
   fmbh_free(t->d_near_box);
   free(t);
@@ -97,6 +99,35 @@ int fmb_tree_build(const fmb_mesh *m, int s, int depth, void *stream,
     if ((rc = fmbh_m2l_pairs(t, l, stream))) goto fail;
   if ((rc = fmbh_near_lists(t, stream))) goto fail;
   if ((rc = fmbh_sorted_geometry(t, d_V, d_F, stream))) goto fail;
+  /* vertex normals: sums of the faces' (n0-n2)x(n1-n2) (area weights) */
+  {
+    double *vn = (double *)calloc((size_t)3 * m->nv, sizeof(double));
+    for (int f = 0; f < n; f++) {
+      const int *id = m->F + 3 * f;
+      const double *P0 = m->V + 3 * id[0], *P1 = m->V + 3 * id[1], *P2 = m->V + 3 * id[2];
+      double a[3], b[3];
+      for (int q = 0; q < 3; q++) {
+        a[q] = P0[q] - P2[q];
+        b[q] = P1[q] - P2[q];
+      }
+      double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                     a[0] * b[1] - a[1] * b[0]};
+      for (int v = 0; v < 3; v++)
+        for (int q = 0; q < 3; q++) vn[3 * id[v] + q] += c[q];
+    }
+    for (int v = 0; v < m->nv; v++) {
+      double s2 = sqrt(vn[3 * v] * vn[3 * v] + vn[3 * v + 1] * vn[3 * v + 1] +
+                       vn[3 * v + 2] * vn[3 * v + 2]);
+      if (s2 > 0)
+        for (int q = 0; q < 3; q++) vn[3 * v + q] /= s2;
+    }
+    t->nvert = m->nv;
+    rc = fmbh_malloc((void **)&t->d_vnorm, sizeof(double) * 3 * m->nv);
+    if (!rc) rc = fmbh_h2d(t->d_vnorm, vn, sizeof(double) * 3 * m->nv, stream);
+    if (!rc) rc = fmbh_sync(stream);
+    free(vn);
+    if (rc) goto fail;
+  }
   {
     int L = t->lmax, nb = t->nbox[L];
     int *bs = (int *)malloc(sizeof(int) * (nb + 1));
@@ -187,4 +218,9 @@ done:
   free(p);
   free(eb);
   return rc;
+}
+
+int fmb_tree_vertex_normals(const fmb_tree *t, double *normals) {
+  if (!t || !normals) return fmb_set_error(FMB_ERR_ARG, "fmb_tree_vertex_normals");
+  return fmbh_d2h(normals, t->d_vnorm, sizeof(double) * 3 * t->nvert, t->stream);
 }

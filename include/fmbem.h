@@ -93,6 +93,10 @@ int fmb_tree_elem_box(const fmb_tree *t, int *box, int *perm);
  * codes (dx+3)*49+(dy+3)*7+(dz+3) of target-minus-source grid steps. */
 int fmb_tree_pairs(const fmb_tree *t, int l, int *npairs, int *tgt, int *src,
                    int *off);
+/* Unit vertex normals (nvert*3 host doubles): normalised sums of the
+ * adjacent faces' (n0-n2)x(n1-n2), i.e. area-weighted face normals.  The
+ * reference has none (used by fmb_adjoint_grad's dJ_dvn). */
+int fmb_tree_vertex_normals(const fmb_tree *t, double *normals);
 void fmb_tree_destroy(fmb_tree *t);
 
 /* ------------------------------------------------------------ operators */
@@ -246,24 +250,32 @@ int fmb_gmres_transpose(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
                         int restart, double tol, int maxit, fmb_gmres_info *info,
                         void *stream);
 
-/* Adjoint gradient (the reference has none).  For the field-point objective
+/* Adjoint gradients (the reference has none).  For the field-point objective
  *   J = sum_r sum_q |p_r(y_q)|^2,
  * with p_r the field of fmb_field_eval from the surface solution x ([nrhs][N]
- * device, A x = b solved beforehand; dirs: nrhs*3 host doubles of the
- * incident plane waves, or NULL for the scattered field) at pts (npts*3 host
- * doubles), returns J (host) and dJ_dsigma[j] = dJ/dRe sigma_j +
- * i dJ/dIm sigma_j ([N] device, caller order) for the plan's Robin
- * coefficients sigma (dp/dn = sigma p).  One adjoint solve A^T lambda =
- * F^T conj(p) (fmb_gmres_transpose with restart/tol/maxit; its report in
- * info, may be NULL), then dJ/dsigma_j = 2 conj(-x_j ((C^T lambda)_j +
- * (Sf^T conj(p))_j)) with C = dA/dsigma's column factor S + alpha K' +
- * alpha I/2 and Sf the single layer at the field points.  For an admittance
- * Y with sigma = i k Y: dJ/dY = -i k dJ/dsigma.  Needs rcr_path
- * FMB_RCR_AUTO. */
+ * device, A x = b solved beforehand for the plane waves dirs: nrhs*3 host
+ * doubles, or NULL for the scattered field) at pts (npts*3 host doubles):
+ *   J (host), may be NULL;
+ *   dJ_dsigma[j] = dJ/dRe sigma_j + i dJ/dIm sigma_j ([N] device, caller
+ *     order) for the plan's Robin coefficients (dp/dn = sigma p); for an
+ *     admittance Y with sigma = i k Y, dJ/dY = -i k dJ/dsigma;
+ *   dJ_dV[3v+a] = dJ/d(vertex v coordinate a) ([nvert*3] device) and
+ *   dJ_dvn[v] = dJ_dV[v] . N_v along the unit area-weighted vertex normal
+ *     N_v (fmb_tree_vertex_normals) ([nvert] device); these need dirs (the
+ *     right-hand side moves with the collocation points).
+ * Any of the three gradient outputs may be NULL, not all.  One adjoint
+ * solve A^T lambda = F^T conj(p) (fmb_gmres_transpose with restart / tol /
+ * maxit; its report in info, may be NULL).  Then
+ *   dJ/dsigma_j = 2 conj(-x_j ((C^T lambda)_j + (Sf^T conj(p))_j)),
+ * C = S + alpha K' + alpha I/2 (dA/dsigma_j's column factor), Sf the single
+ * layer at the field points; and dJ = 2 Re [conj(p)^T dF x + lambda^T (db -
+ * dA x)] per element, the element's own panel, collocation point, normal
+ * and self term moved by central differences of its three vertices.  Needs
+ * rcr_path FMB_RCR_AUTO. */
 int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dirs,
                      const double *pts, int npts, int restart, double tol, int maxit,
-                     double *J, fmb_c64 *dJ_dsigma, fmb_gmres_info *info,
-                     void *stream);
+                     double *J, fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
+                     fmb_gmres_info *info, void *stream);
 /* Precision of the Arnoldi matvecs of fmb_gmres on this plan (FMB_MIXED only
  * on a plan created with FMB_MIXED).  The reference has no solver. */
 int fmb_plan_set_work_precision(fmb_plan *pl, int prec);

@@ -151,3 +151,174 @@ def test_autograd_admittance_objective(sphere_l3):
     lin = np.real(np.sum(np.conj(g) * dY))
     assert abs(lin - fd) <= 1e-5 * abs(fd), (lin, fd)
     assert abs(J.item() - _fmm_objective(pl, k, Yn, pts, d)) < 1e-10 * J.item()
+
+
+def _objective_on_mesh(V, Fc, k, Y, pts, d, s=5, tol=1e-13):
+    """J with the FMM operator of a tree and plan built on the mesh (V, Fc)."""
+    m = F.Mesh.from_arrays(V, Fc)
+    pl = F.Plan(F.Tree(m, s=s), k, robin=torch.tensor(1j * k * Y, device="cuda"))
+    x, _ = pl.gmres(pl.rhs_plane_wave(d), tol=tol)
+    p = pl.field(x, pts, d).cpu().numpy()[0]
+    return np.sum(np.abs(p) ** 2)
+
+
+def _generic_rotation(V, Fc, gap=2e-6):
+    """A seeded rotation of the mesh after which no centroid coordinate lies
+    within `gap` of a box plane of its octree at any level: the icosphere's
+    symmetric centroids sit exactly on the midplanes, and a displacement
+    would move them between boxes (a discontinuity of the FMM operator, not
+    a derivative)."""
+    for seed in range(50):
+        Q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+        W = V @ Q.T
+        c = W[Fc].mean(axis=1)
+        ot = O.Tree(W, Fc, s=5)
+        D, pmin = ot.geom()
+        ok = True
+        for l in range(2, ot.lmax + 1):
+            hb = D / 2 ** l
+            rel = (c - np.asarray(pmin)) / hb
+            near = np.round(rel)
+            inner = (near > 0) & (near < 2 ** l)  # the cube's faces hold the fixed extremes
+            if (np.abs(rel - near) * hb)[inner].min() < gap:
+                ok = False
+                break
+        if ok:
+            return W
+    raise AssertionError("no generic rotation found")
+
+
+def test_vertex_gradients(sphere_l3):
+    """dJ/dV and dJ/d(vertex normal displacement) on the 1,280-triangle
+    sphere (rotated generically) at 4 kHz with a random admittance: the
+    directional derivative along a random normal displacement field
+    phi_v N_v against central differences (step 1e-6 m) of J with the FMM
+    operator rebuilt on the displaced mesh (1e-5 relative), and against those
+    of the oracle's dense objective (FMM-level agreement); dJ_dvn =
+    dJ_dV . N_v; FP32 Arnoldi (mixed plan) gives the same gradient."""
+    _, V0, Fc = sphere_l3
+    V = _generic_rotation(V0, Fc)
+    m = F.Mesh.from_arrays(V, Fc)
+    n, nv = Fc.shape[0], V.shape[0]
+    k = wavenumber(4000.0)
+    Y = admittance(n, 12)
+    pts = circle_points()
+    d = np.array([[0.0, 0.0, 1.0]])
+    t = F.Tree(m, s=5)
+    Nv = t.vertex_normals()
+    np.testing.assert_allclose(np.linalg.norm(Nv, axis=1), 1.0, atol=1e-12)
+    assert np.all(np.sum(Nv * V, axis=1) > 0)  # outward on the sphere
+    grads = {}
+    for prec in (F.FMB_FP64, F.FMB_MIXED):
+        pl = F.Plan(t, k, robin=torch.tensor(1j * k * Y, device="cuda"), precision=prec)
+        x, _ = pl.gmres(pl.rhs_plane_wave(d), tol=1e-12)
+        J, gs, info, dV, dvn = pl.adjoint_grad(x, pts, d, tol=1e-12, vertices=True)
+        grads[prec] = (J, dV.cpu().numpy(), dvn.cpu().numpy())
+    J, dV, dvn = grads[F.FMB_FP64]
+    np.testing.assert_allclose(dvn, np.sum(dV * Nv, axis=1), rtol=1e-12, atol=1e-14 * np.abs(dvn).max())
+    _, dV32, dvn32 = grads[F.FMB_MIXED]
+    assert np.linalg.norm(dvn32 - dvn) / np.linalg.norm(dvn) < 1e-6
+    # displacements vanish on the elements whose centroids bound the tree's
+    # cube, so the rebuilt tree keeps its boxes (the adjoint differentiates
+    # the FMM operator of a fixed tree)
+    cen = V[Fc].mean(axis=1)
+    fixed = np.zeros(nv, bool)
+    for a in range(3):
+        for ext in (cen[:, a].min(), cen[:, a].max()):
+            fixed[Fc[np.abs(cen[:, a] - ext) < 1e-4].ravel()] = True
+    phi = np.random.default_rng(13).uniform(-1, 1, nv) * ~fixed
+    h = 1e-6
+    Jp = _objective_on_mesh(V + h * phi[:, None] * Nv, Fc, k, Y, pts, d)
+    Jm = _objective_on_mesh(V - h * phi[:, None] * Nv, Fc, k, Y, pts, d)
+    fd = (Jp - Jm) / (2 * h)
+    lin = np.sum(dvn * phi)
+    print("vertex-normal directional derivative: adjoint %.10e  fd %.10e" % (lin, fd))
+    assert abs(lin - fd) <= 1e-5 * abs(fd), (lin, fd)
+    # a tangential-plus-normal direction through dJ_dV
+    dir3 = np.random.default_rng(14).uniform(-1, 1, (nv, 3)) * ~fixed[:, None]
+    Jp = _objective_on_mesh(V + h * dir3, Fc, k, Y, pts, d)
+    Jm = _objective_on_mesh(V - h * dir3, Fc, k, Y, pts, d)
+    fd3 = (Jp - Jm) / (2 * h)
+    lin3 = np.sum(dV * dir3)
+    assert abs(lin3 - fd3) <= 1e-5 * abs(fd3), (lin3, fd3)
+    # the oracle's dense objective, same displacement (step 1e-6 m)
+    sig = 1j * k * Y
+
+    def dense_J(Vd):
+        b = O.Bem(Vd, Fc, k, sigma=sig)
+        xd = np.linalg.solve(b.dense_matrix(), b.rhs_plane(d[0]))
+        return np.sum(np.abs(b.field(xd, pts, dirs=d)[0]) ** 2)
+
+    fdd = (dense_J(V + h * phi[:, None] * Nv) - dense_J(V - h * phi[:, None] * Nv)) / (2 * h)
+    print("dense-oracle directional derivative %.10e" % fdd)
+    assert abs(lin - fdd) <= 2e-3 * abs(fdd), (lin, fdd)
+
+
+def _movable(t, V, Fc, gap=2e-6):
+    """Vertices that may move without any element changing its octree box
+    or the tree its cube: those of elements whose centroids are at least
+    `gap` from every box plane and not among the extreme centroids."""
+    _, L, D, pmin = t.info()
+    c = V[Fc].mean(axis=1)
+    bad = np.zeros(len(Fc), bool)
+    for l in range(2, L + 1):
+        hb = D / 2 ** l
+        rel = (c - np.asarray(pmin)) / hb
+        bad |= (np.abs(rel - np.round(rel)) * hb < gap).any(axis=1)
+    for a in range(3):
+        for ext in (c[:, a].min(), c[:, a].max()):
+            bad |= np.abs(c[:, a] - ext) < 1e-4
+    fixed = np.zeros(len(V), bool)
+    fixed[Fc[bad].ravel()] = True
+    return ~fixed
+
+
+def test_adjoint_config4_sphere_20480():
+    """The differentiation configuration: rigid-sphere mesh r = 0.1 m,
+    icosphere level 5 (20,480 triangles), 4 kHz, plane wave along +z,
+    per-triangle admittance Y = 0.01 (1 + 0.5 U(-1,1)) + 0.005i U(-1,1)
+    (seeded), J = sum |p|^2 at 32 points on a 0.5 m circle.  dJ/dY and
+    dJ/d(vertex normal displacement) in FP64 and mixed form against central
+    differences (step 1e-6) of the FMM objective: 1e-5 relative."""
+    import time
+    m = F.Mesh.icosphere(5, 0.1)
+    V, Fc = m.arrays()
+    n = len(Fc)
+    k = wavenumber(4000.0)
+    Y = admittance(n, 2024)
+    pts = circle_points()
+    d = np.array([[0.0, 0.0, 1.0]])
+    t = F.Tree(m, s=5)
+    Nv = t.vertex_normals()
+    out = {}
+    for prec in (F.FMB_FP64, F.FMB_MIXED):
+        pl = F.Plan(t, k, robin=torch.tensor(1j * k * Y, device="cuda"), precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, info = pl.gmres(pl.rhs_plane_wave(d), tol=1e-12)
+        t1 = time.perf_counter()
+        J, gs, ainfo, dV, dvn = pl.adjoint_grad(x, pts, d, tol=1e-12, vertices=True)
+        t2 = time.perf_counter()
+        print("config 4 %s: forward solve %.3fs (%d its), adjoint + gradients %.3fs (%d its), J %.12e"
+              % ("fp64" if prec == F.FMB_FP64 else "mixed", t1 - t0, info.iters, t2 - t1,
+                 ainfo.iters, J))
+        out[prec] = (J, (-1j * k * gs).cpu().numpy(), dvn.cpu().numpy(), pl)
+    J, gY, dvn, pl = out[F.FMB_FP64]
+    _, gY32, dvn32, _ = out[F.FMB_MIXED]
+    assert np.linalg.norm(gY32 - gY) / np.linalg.norm(gY) < 1e-6
+    assert np.linalg.norm(dvn32 - dvn) / np.linalg.norm(dvn) < 1e-6
+    h = 1e-6
+    # admittance: random complex direction
+    dY = _rand(n, 77)
+    fd = (_fmm_objective(pl, k, Y + h * dY, pts, d) - _fmm_objective(pl, k, Y - h * dY, pts, d)) / (2 * h)
+    lin = np.real(np.sum(np.conj(gY) * dY))
+    print("config 4 dJ/dY directional: adjoint %.10e fd %.10e" % (lin, fd))
+    assert abs(lin - fd) <= 1e-5 * abs(fd)
+    # vertex normal displacement: random field on the movable vertices
+    phi = np.random.default_rng(78).uniform(-1, 1, len(V)) * _movable(t, V, Fc)
+    Jp = _objective_on_mesh(V + h * phi[:, None] * Nv, Fc, k, Y, pts, d)
+    Jm = _objective_on_mesh(V - h * phi[:, None] * Nv, Fc, k, Y, pts, d)
+    fdv = (Jp - Jm) / (2 * h)
+    linv = np.sum(dvn * phi)
+    print("config 4 dJ/dvn directional: adjoint %.10e fd %.10e" % (linv, fdv))
+    assert abs(linv - fdv) <= 1e-5 * abs(fdv)
