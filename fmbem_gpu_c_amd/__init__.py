@@ -48,6 +48,7 @@ class PlanParams(ctypes.Structure):
         ("nrhs_max", ctypes.c_int),
         ("rcr_path", ctypes.c_int),
         ("k_max", ctypes.c_double),
+        ("autotune", ctypes.c_int),
     ]
 
 
@@ -112,6 +113,12 @@ _SIGS = {
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp,
                                         _vp, _vp, _vp, _vp, _vp]),
     "fmb_tree_vertex_normals": (ctypes.c_int, [_vp, _vp]),
+    "fmb_matvec_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _vp, _vp, ctypes.c_int, _vp]),
+    "fmb_gmres_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    "fmb_plan_tile_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           _vp, _vp, _vp, _vp]),
     "fmb_plan_wavenumber": (ctypes.c_int, [_vp, _vp]),
     "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -327,6 +334,16 @@ class Plan:
                                      nrhs, _dev(x0), _stream()))
         return x0
 
+    def tile_shape(self, kind, level, prec=FMB_FP64):
+        """(mode, nt, mt, tuned ms) of a matrix-core translation
+        (fmb_plan_tile_shape; kind 0 M2L, 1 M2M, 2 L2L)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        ms = ctypes.c_double()
+        _check(lib().fmb_plan_tile_shape(self.h, kind, level, prec, ctypes.byref(v[0]),
+                                         ctypes.byref(v[1]), ctypes.byref(v[2]),
+                                         ctypes.byref(ms)))
+        return v[0].value, v[1].value, v[2].value, ms.value
+
     def orders(self):
         p = np.zeros(16, dtype=np.int32)
         _check(lib().fmb_plan_orders(self.h, _ptr(p, ctypes.c_int)))
@@ -486,6 +503,37 @@ def gen_coax_mats(kind, k, t, p):
     _check(lib().fmb_gen_coax_mats(kind, k, _ptr(t, ctypes.c_double), t.shape[0], p,
                                    out.ctypes.data_as(ctypes.c_void_p)))
     return out
+
+
+def matvec_batch(plans, nrhs, x, graph=True, prec=FMB_FP64, reps=1):
+    """fmb_matvec_batch: every plan applied to its columns of x
+    ([sum(nrhs)][N]); returns [reps][sum(nrhs)][N] (one slice per replay)."""
+    import torch
+    nrhs = [int(r) for r in nrhs]
+    arr = (_vp * len(plans))(*[p.h for p in plans])
+    nr = (ctypes.c_int * len(plans))(*nrhs)
+    y = torch.empty((reps,) + tuple(x.shape), dtype=x.dtype, device=x.device)
+    _check(lib().fmb_matvec_batch(arr, nr, len(plans), int(graph), prec, _dev(x), _dev(y),
+                                  reps, _stream()))
+    return y
+
+
+def gmres_batch(plans, nrhs, b, x0=None, graph=True, restart=30, tol=1e-6, maxit=1000,
+                check=True):
+    """fmb_gmres_batch: GMRES over several plans (frequencies) in lock step.
+    b: complex128 CUDA tensor [sum(nrhs)][N], plan 0's columns first.
+    Returns (x, list of GmresInfo, one per plan)."""
+    import torch
+    nrhs = [int(r) for r in nrhs]
+    arr = (_vp * len(plans))(*[p.h for p in plans])
+    nr = (ctypes.c_int * len(plans))(*nrhs)
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    infos = (GmresInfo * len(plans))()
+    rc = lib().fmb_gmres_batch(arr, nr, len(plans), int(graph), _dev(b), _dev(x), restart,
+                               tol, maxit, infos, _stream())
+    if check:
+        _check(rc)
+    return x, list(infos)
 
 
 def _admittance_function():

@@ -65,6 +65,7 @@ typedef struct fmb_opset {
   void *d_M[FMB_MAXLEV], *d_L[FMB_MAXLEV]; /* [nbox][nrhs][p^2] complex      */
   void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
   void *d_gws;       /* global workspace of the GWS-mode translations       */
+  size_t gws_bytes;
   void *d_Hpad;      /* zero-padded rotation blocks [nbeta][hpad_stride]    */
   void *d_phase[3];  /* e^{i m phi}, m < Pmax: M2L offsets, M2M, L2L octants */
   void *d_phase_c[3]; /* e^{-i m phi}: the same buckets, transposed pass    */
@@ -82,6 +83,7 @@ typedef struct fmb_rcr_tab {
   int *d_sched[2];      /* per opset: unit schedule                        */
   int sched_len[2];     /* ints in d_sched                                 */
   void *d_Cpad[2];      /* per opset: zero-padded coaxial blocks           */
+  float tuned_ms[2];    /* per opset: autotuned time of one launch (0: not tuned) */
   size_t cpad_stride;   /* complex values per coaxial matrix in d_Cpad     */
 } fmb_rcr_tab;
 

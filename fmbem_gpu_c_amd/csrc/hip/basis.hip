@@ -124,7 +124,7 @@ static int launch_bases(fmb_plan *pl, int op, const fmb_c64 *robin, void *P2M,
     if (lds > 64 * 1024)
       FMB_HIP_CHECK(hipFuncSetAttribute(
           (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-          (int)lds));
+          160 * 1024));
     hipLaunchKernelGGL(kern, dim3(t->nelem), dim3(WG), lds, st, t->nelem, pL,
                        pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
                        t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],
@@ -136,7 +136,7 @@ static int launch_bases(fmb_plan *pl, int op, const fmb_c64 *robin, void *P2M,
     if (lds > 64 * 1024)
       FMB_HIP_CHECK(hipFuncSetAttribute(
           (const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-          (int)lds));
+          160 * 1024));
     hipLaunchKernelGGL(kern, dim3(t->nelem), dim3(WG), lds, st, t->nelem, pL,
                        pl->k, pl->alpha_re, pl->alpha_im, t->d_tri, t->d_geo,
                        t->d_elem_box, t->d_keys[L], L, t->D, t->pmin[0],

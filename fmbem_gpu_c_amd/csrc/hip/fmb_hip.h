@@ -27,6 +27,13 @@ int fmbh_event_create(void **ev);
 int fmbh_event_record(void *ev, void *stream);
 int fmbh_event_elapsed(void *ev0, void *ev1, float *ms);
 int fmbh_event_destroy(void *ev);
+int fmbh_stream_create(void **s);
+int fmbh_stream_destroy(void *s);
+int fmbh_stream_wait(void *stream, void *ev);
+int fmbh_capture_begin(void *stream);
+int fmbh_capture_end(void *stream, void **exec);
+int fmbh_graph_launch(void *exec, void *stream);
+int fmbh_graph_destroy(void *exec);
 
 /* ---- tree (tree.hip) */
 int fmbh_centroid_bbox(const double *d_V, const int *d_F, int n,
@@ -98,6 +105,8 @@ typedef struct fmbh_rcr_job {
  * leaves room for a schedule of up to 8 pm + 16 ints) */
 int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt);
 /* global workspace bytes of one workgroup in GWS mode */
+size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len);
+int fmbh_rcr_mt_lds(int pm);
 size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt);
 int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
 /* zero-padded operator copies: rotation block n at hoff[n] (16-row x

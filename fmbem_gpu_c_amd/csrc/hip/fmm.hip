@@ -280,10 +280,10 @@ template <typename K>
 int set_lds(K kern, size_t bytes) {
   if (bytes > 160 * 1024)
     return fmb_set_error(FMB_ERR_RANGE, "translation needs %zu B of LDS", bytes);
-  if (bytes > 64 * 1024)
+  if (bytes > 64 * 1024) /* the CU's 160 KB: valid for every captured launch */
     FMB_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)bytes));
+                                      160 * 1024));
   return FMB_OK;
 }
 

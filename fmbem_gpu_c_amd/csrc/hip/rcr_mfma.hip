@@ -457,12 +457,17 @@ size_t x_bytes(int pm, int nt, size_t tsz) {
   return (size_t)pm * pm * (16 * nt + 2) * tsz;
 }
 
+// Raises the kernel's dynamic-LDS cap to the CU's 160 KB (not to this
+// launch's size): launches captured into a graph replay after every capture
+// has run, so a cap lowered by a later, smaller launch would invalidate them.
 template <typename K>
 int set_lds(K kern, size_t lds) {
+  if (lds > 160 * 1024)
+    return fmb_set_error(FMB_ERR_RANGE, "translation batch needs %zu B of LDS", lds);
   if (lds > 64 * 1024)
     FMB_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+                                      160 * 1024));
   return FMB_OK;
 }
 
@@ -541,6 +546,16 @@ extern "C" int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt) {
   }
   return FMB_OK;
 }
+
+/* LDS-mode bytes of a workgroup (0: the block rows need more than 6 tiles,
+ * the largest LDS-mode unit) */
+extern "C" size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len) {
+  const size_t tsz = prec == FMB_FP64 ? sizeof(double) : sizeof(float);
+  if (!mt_lds(pm)) return 0;
+  return x_bytes(pm, nt, tsz) + small_lds(pm, tsz, sched_len);
+}
+
+extern "C" int fmbh_rcr_mt_lds(int pm) { return mt_lds(pm); }
 
 extern "C" size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt) {
   const size_t tsz = prec == FMB_FP64 ? sizeof(double) : sizeof(float);

@@ -101,3 +101,51 @@ extern "C" int fmbh_event_destroy(void *ev) {
   if (ev) FMB_HIP_CHECK(hipEventDestroy((hipEvent_t)ev));
   return FMB_OK;
 }
+
+/* streams (default priority) and graph capture of a stream's work */
+extern "C" int fmbh_stream_create(void **s) {
+  hipStream_t st;
+  FMB_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *s = st;
+  return FMB_OK;
+}
+
+extern "C" int fmbh_stream_destroy(void *s) {
+  if (s) FMB_HIP_CHECK(hipStreamDestroy((hipStream_t)s));
+  return FMB_OK;
+}
+
+extern "C" int fmbh_stream_wait(void *stream, void *ev) {
+  FMB_HIP_CHECK(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0));
+  return FMB_OK;
+}
+
+extern "C" int fmbh_capture_begin(void *stream) {
+  FMB_HIP_CHECK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+  return FMB_OK;
+}
+
+/* ends the capture; *exec receives the instantiated graph (NULL on error,
+ * the capture being closed either way) */
+extern "C" int fmbh_capture_end(void *stream, void **exec) {
+  hipGraph_t g = NULL;
+  *exec = NULL;
+  FMB_HIP_CHECK(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t e = NULL;
+  hipError_t err = hipGraphInstantiate(&e, g, NULL, NULL, 0);
+  hipGraphDestroy(g);
+  if (err != hipSuccess)
+    return fmb_set_error(FMB_ERR_HIP, "graph instantiation: %s", hipGetErrorString(err));
+  *exec = e;
+  return FMB_OK;
+}
+
+extern "C" int fmbh_graph_launch(void *exec, void *stream) {
+  FMB_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream));
+  return FMB_OK;
+}
+
+extern "C" int fmbh_graph_destroy(void *exec) {
+  if (exec) FMB_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)exec));
+  return FMB_OK;
+}

@@ -1,5 +1,14 @@
 /* gmres.c -- restarted GMRES (Saad & Schultz 1986) on the device, batched
- * over right-hand sides in lock step.  The reference has no solver.
+ * in lock step over right-hand sides and, with fmb_gmres_batch, over plans
+ * (frequencies) that share the element count.  The reference has no solver.
+ *
+ * A batch applies every plan per Arnoldi step; with graph mode the plans'
+ * FMM passes are captured once per precision into one hipGraph (one branch
+ * per plan, forked streams) between fixed staging vectors.  A plan retires
+ * as soon as its estimated residuals meet the tolerance (its columns of x
+ * updated at once and its FP64 true residual checked) or its true residual
+ * does at a restart; retired plans leave the pass and the graph is captured
+ * again.
  *
  * Arnoldi vectors, inner products and the least-squares update are FP64
  * (classical Gram-Schmidt with one reorthogonalisation pass; complex Givens
@@ -31,33 +40,179 @@ static fmb_c64 fromz(zc z) {
   return v;
 }
 
-/* y = A x, or A^T x for the transposed system */
-static int apply(fmb_plan *pl, int trans, int prec, const fmb_c64 *x, fmb_c64 *y,
-                 int nrhs, void *stream) {
+/* The system: plan p owns columns [off[p], off[p+1]) of every vector. */
+typedef struct {
+  fmb_plan *const *pl;
+  const int *nr;
+  int np, trans, N, graph;
+  int *off;
+  int *live;          /* plans in the captured graphs                  */
+  void *exec[2];      /* graph per precision (0 FP64, 1 working)       */
+  void *cs, **bs, *ev_fork, **ev_join;
+  fmb_c64 *xs, *ys;   /* staging input; output (= the work vector)     */
+} sys_t;
+
+static int plan_apply(fmb_plan *pl, int trans, int prec, const fmb_c64 *x, fmb_c64 *y,
+                      int nrhs, void *stream) {
   if (!trans) return fmb_matvec(pl, prec, x, y, nrhs, stream);
   int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
   return fmbx_matvec_t(pl, op, 3, NULL, NULL, x, y, nrhs, stream);
 }
 
-static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
-                     int nrhs, int restart, double tol, int maxit,
-                     fmb_gmres_info *info, void *stream) {
-  if (!pl || !pl->ready || !b || !x || nrhs < 1 || nrhs > pl->nrhs_max || restart < 1 ||
-      !(tol > 0) || maxit < 1)
+static void graphs_drop(sys_t *S) {
+  for (int i = 0; i < 2; i++) {
+    fmbh_graph_destroy(S->exec[i]);
+    S->exec[i] = NULL;
+  }
+}
+
+/* one branch per live plan from xs to ys, forked from the capture stream */
+static int graph_capture(sys_t *S, int prec, void **exec) {
+  int rc = fmbh_capture_begin(S->cs), rc2;
+  if (!rc) rc = fmbh_event_record(S->ev_fork, S->cs);
+  for (int p = 0; p < S->np && !rc; p++) {
+    if (!S->live[p]) continue;
+    fmb_plan *pl = S->pl[p];
+    int tm = pl->timing;
+    pl->timing = 0;
+    size_t o = (size_t)S->off[p] * S->N;
+    rc = fmbh_stream_wait(S->bs[p], S->ev_fork);
+    if (!rc) rc = plan_apply(pl, S->trans, prec, S->xs + o, S->ys + o, S->nr[p], S->bs[p]);
+    pl->timing = tm;
+    if (!rc) rc = fmbh_event_record(S->ev_join[p], S->bs[p]);
+    if (!rc) rc = fmbh_stream_wait(S->cs, S->ev_join[p]);
+  }
+  rc2 = fmbh_capture_end(S->cs, exec);
+  return rc ? rc : rc2;
+}
+
+/* y = A x (A^T x), every plan on its columns; y must be S->ys in graph mode */
+static int apply(sys_t *S, int prec, const fmb_c64 *x, fmb_c64 *y, void *stream) {
+  if (!S->graph) {
+    for (int p = 0; p < S->np; p++) {
+      if (!S->live[p]) continue;
+      size_t o = (size_t)S->off[p] * S->N;
+      int rc = plan_apply(S->pl[p], S->trans, prec, x + o, y + o, S->nr[p], stream);
+      if (rc) return rc;
+    }
+    return FMB_OK;
+  }
+  int i = prec == FMB_MIXED ? 1 : 0, rc;
+  if (!S->exec[i] && (rc = graph_capture(S, prec, &S->exec[i]))) return rc;
+  if (x != S->xs &&
+      (rc = fmbh_d2d(S->xs, x, sizeof(fmb_c64) * (size_t)S->off[S->np] * S->N, stream)))
+    return rc;
+  return fmbh_graph_launch(S->exec[i], stream);
+}
+
+/* validates the plans of a system and fills S (columns, live plans) */
+static int sys_init(sys_t *S, fmb_plan *const *pls, const int *nrs, int np, int trans,
+                    int graph) {
+  memset(S, 0, sizeof(*S));
+  if (!pls || !nrs || np < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
-  if (trans && pl->prm.rcr_path == FMB_RCR_VALU)
-    return fmb_set_error(FMB_ERR_ARG, "transposed solve needs rcr_path FMB_RCR_AUTO");
+  for (int p = 0; p < np; p++) {
+    const fmb_plan *q = pls[p];
+    if (!q || !q->ready || nrs[p] < 1 || nrs[p] > q->nrhs_max ||
+        q->tree->nelem != pls[0]->tree->nelem || q->work_prec != pls[0]->work_prec)
+      return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: plan %d does not fit the batch", p);
+    if (trans && q->prm.rcr_path == FMB_RCR_VALU)
+      return fmb_set_error(FMB_ERR_ARG, "transposed solve needs rcr_path FMB_RCR_AUTO");
+  }
+  S->pl = pls;
+  S->nr = nrs;
+  S->np = np;
+  S->trans = trans;
+  S->graph = graph;
+  S->N = pls[0]->tree->nelem;
+  S->off = (int *)malloc(sizeof(int) * (np + 1));
+  S->live = (int *)malloc(sizeof(int) * np);
+  S->off[0] = 0;
+  for (int p = 0; p < np; p++) {
+    S->off[p + 1] = S->off[p] + nrs[p];
+    S->live[p] = 1;
+  }
+  return FMB_OK;
+}
+
+/* graph-mode resources: staging input, the output vector ys, streams */
+static int sys_graph_init(sys_t *S, fmb_c64 *ys) {
+  int rc = fmbh_malloc((void **)&S->xs, sizeof(fmb_c64) * (size_t)S->off[S->np] * S->N);
+  if (rc) return rc;
+  S->ys = ys;
+  S->bs = (void **)calloc(S->np, sizeof(void *));
+  S->ev_join = (void **)calloc(S->np, sizeof(void *));
+  if ((rc = fmbh_stream_create(&S->cs))) return rc;
+  if ((rc = fmbh_event_create(&S->ev_fork))) return rc;
+  for (int p = 0; p < S->np; p++) {
+    if ((rc = fmbh_stream_create(&S->bs[p]))) return rc;
+    if ((rc = fmbh_event_create(&S->ev_join[p]))) return rc;
+  }
+  return FMB_OK;
+}
+
+static void sys_free(sys_t *S) {
+  if (S->cs) fmbh_sync(S->cs);
+  graphs_drop(S);
+  for (int p = 0; p < S->np; p++) {
+    if (S->bs) fmbh_stream_destroy(S->bs[p]);
+    if (S->ev_join) fmbh_event_destroy(S->ev_join[p]);
+  }
+  fmbh_stream_destroy(S->cs);
+  fmbh_event_destroy(S->ev_fork);
+  fmbh_free(S->xs);
+  free(S->bs);
+  free(S->ev_join);
+  free(S->off);
+  free(S->live);
+  memset(S, 0, sizeof(*S));
+}
+
+/* x += V y over the first j Arnoldi vectors, y from the triangular solve of
+ * the rotated Hessenberg systems, for the columns with sel[r] (others y = 0) */
+static int update_x(const zc *Hh, const zc *g, zc *yv, fmb_c64 *hh, int m, int nrhs,
+                    int j, const int *sel, const fmb_c64 *dV, fmb_c64 *dy, fmb_c64 *x,
+                    int N, void *stream) {
+#define HQ(r, i, c) Hh[((size_t)(r) * m + (c)) * (m + 1) + (i)]
+  for (int r = 0; r < nrhs; r++) {
+    for (int q = j - 1; q >= 0; q--) {
+      zc s = g[(size_t)r * (m + 1) + q];
+      for (int c2 = q + 1; c2 < j; c2++) s -= HQ(r, q, c2) * yv[(size_t)r * (m + 1) + c2];
+      zc d = HQ(r, q, q);
+      yv[(size_t)r * (m + 1) + q] = (sel[r] && cabs(d) > 0) ? s / d : 0.0;
+    }
+    for (int q = 0; q < j; q++) hh[(size_t)q * nrhs + r] = fromz(yv[(size_t)r * (m + 1) + q]);
+  }
+#undef HQ
+  int rc = fmbh_h2d(dy, hh, sizeof(fmb_c64) * j * nrhs, stream);
+  if (!rc) rc = fmbh_zgemv_add(dV, j, dy, x, nrhs, N, stream);
+  if (!rc) rc = fmbh_sync(stream); /* hh is reused at once */
+  return rc;
+}
+
+static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, int graph,
+                     const fmb_c64 *b, fmb_c64 *x, int restart, double tol, int maxit,
+                     fmb_gmres_info *info, void *stream) {
+  if (!b || !x || restart < 1 || !(tol > 0) || maxit < 1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
+  sys_t S;
+  int rc = sys_init(&S, pls, nrs, np, trans, graph);
+  if (rc) {
+    sys_free(&S);
+    return rc;
+  }
+  const fmb_plan *pl = pls[0];
+  const int nrhs = S.off[np];
   double t0 = now_s();
   const int N = pl->tree->nelem, m = restart;
   const int work = pl->work_prec;
   size_t vbytes = sizeof(fmb_c64) * (size_t)N * nrhs;
-  int rc;
   fmb_c64 *dV = NULL, *dw = NULL, *dr = NULL, *dh = NULL, *dh2 = NULL, *dy = NULL;
   double *dn = NULL;
   zc *Hh = NULL, *cs = NULL, *sn = NULL, *g = NULL, *yv = NULL;
   fmb_c64 *hh = NULL, *hh2 = NULL;
-  double *bn = NULL, *rn = NULL, *hn = NULL;
-  int *active = NULL;
+  double *bn = NULL, *rn = NULL, *hn = NULL, *rsave = NULL;
+  int *active = NULL, *sel = NULL;
   int iters = 0, mv64 = 0, mvw = 0, restarts = 0, conv = 0;
   if ((rc = fmbh_malloc((void **)&dV, vbytes * (m + 1)))) goto done;
   if ((rc = fmbh_malloc((void **)&dw, vbytes))) goto done;
@@ -66,6 +221,7 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
   if ((rc = fmbh_malloc((void **)&dh2, sizeof(fmb_c64) * (m + 1) * nrhs))) goto done;
   if ((rc = fmbh_malloc((void **)&dy, sizeof(fmb_c64) * (m + 1) * nrhs))) goto done;
   if ((rc = fmbh_malloc((void **)&dn, sizeof(double) * nrhs))) goto done;
+  if (graph && (rc = sys_graph_init(&S, dw))) goto done;
   Hh = (zc *)calloc((size_t)(m + 1) * m * nrhs, sizeof(zc));
   cs = (zc *)calloc((size_t)m * nrhs, sizeof(zc));
   sn = (zc *)calloc((size_t)m * nrhs, sizeof(zc));
@@ -76,7 +232,9 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
   bn = (double *)malloc(sizeof(double) * nrhs);
   rn = (double *)malloc(sizeof(double) * nrhs);
   hn = (double *)malloc(sizeof(double) * nrhs);
+  rsave = (double *)malloc(sizeof(double) * nrhs);
   active = (int *)malloc(sizeof(int) * nrhs);
+  sel = (int *)malloc(sizeof(int) * nrhs);
 #define HH(r, i, j) Hh[((size_t)(r) * m + (j)) * (m + 1) + (i)]
   if ((rc = fmbh_znorms(b, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
@@ -84,7 +242,7 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
     if (bn[r] == 0.0) bn[r] = 1.0;
   while (1) {
     /* true residual r = b - A x in FP64 */
-    if ((rc = apply(pl, trans, FMB_FP64, x, dw, nrhs, stream))) goto done;
+    if ((rc = apply(&S, FMB_FP64, x, dw, stream))) goto done;
     mv64++;
     {
       fmb_c64 *one = hh, *mone = hh2;
@@ -102,12 +260,31 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
     }
     if ((rc = fmbh_znorms(dr, nrhs, N, dn, stream))) goto done;
     if ((rc = fmbh_d2h(rn, dn, sizeof(double) * nrhs, stream))) goto done;
+    /* plans out of the graphs are not applied any more: their residuals are
+     * the ones they converged with (their x no longer changes) */
+    for (int p = 0; p < np; p++)
+      if (!S.live[p])
+        for (int r = S.off[p]; r < S.off[p + 1]; r++) rn[r] = rsave[r];
     conv = 1;
     for (int r = 0; r < nrhs; r++) {
       active[r] = rn[r] / bn[r] > tol;
       if (active[r]) conv = 0;
     }
     if (conv || iters >= maxit) break;
+    if (np > 1)
+      for (int p = 0; p < np; p++) {
+        int any = 0;
+        for (int r = S.off[p]; r < S.off[p + 1]; r++) any |= active[r];
+        if (S.live[p] && !any) {
+          /* converged plan: out of the graphs, its columns of w held at 0 */
+          S.live[p] = 0;
+          for (int r = S.off[p]; r < S.off[p + 1]; r++) rsave[r] = rn[r];
+          graphs_drop(&S);
+          size_t o = (size_t)S.off[p] * S.N;
+          if ((rc = fmbh_memset(dw + o, 0, sizeof(fmb_c64) * nrs[p] * (size_t)S.N, stream)))
+            goto done;
+        }
+      }
     if (restarts > 0 || iters > 0) restarts++;
     /* V_0 = r / |r| */
     for (int r = 0; r < nrhs; r++) {
@@ -122,7 +299,7 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
     int j;
     for (j = 0; j < m && iters < maxit; j++) {
       fmb_c64 *Vj = dV + (size_t)j * N * nrhs;
-      if ((rc = apply(pl, trans, work, Vj, dw, nrhs, stream))) goto done;
+      if ((rc = apply(&S, work, Vj, dw, stream))) goto done;
       if (work == FMB_MIXED)
         mvw++;
       else
@@ -170,39 +347,77 @@ static int gmres_run(fmb_plan *pl, int trans, const fmb_c64 *b, fmb_c64 *x,
       if ((rc = fmbh_zaxpby(dh, dw, NULL, dV + (size_t)(j + 1) * N * nrhs, nrhs,
                             N, stream)))
         goto done;
+      /* a batch retires a plan as soon as its estimated residuals meet the
+       * tolerance: its columns of x take this step's update, its FP64 true
+       * residual is checked, and it leaves the pass (a plan that fails the
+       * check rejoins at the next restart) */
+      for (int p = 0; p < np && np > 1 && !all_done; p++) {
+        int any = 0, ok = 1;
+        for (int r = S.off[p]; r < S.off[p + 1]; r++)
+          if (active[r]) {
+            any = 1;
+            if (cabs(g[(size_t)r * (m + 1) + j + 1]) / bn[r] > tol) ok = 0;
+          }
+        if (!S.live[p] || !any || !ok) continue;
+        for (int r = 0; r < nrhs; r++)
+          sel[r] = r >= S.off[p] && r < S.off[p + 1] && active[r];
+        if ((rc = update_x(Hh, g, yv, hh, m, nrhs, j + 1, sel, dV, dy, x, N, stream)))
+          goto done;
+        for (int r = S.off[p]; r < S.off[p + 1]; r++) active[r] = 0;
+        const size_t o = (size_t)S.off[p] * N;
+        const int nq = nrs[p];
+        /* w's columns of p are free (V_{j+1} is formed): A x_p there */
+        if ((rc = plan_apply(S.pl[p], trans, FMB_FP64, x + o, dw + o, nq, stream))) goto done;
+        mv64++;
+        if ((rc = fmbh_d2d(dr + o, b + o, sizeof(fmb_c64) * nq * (size_t)N, stream))) goto done;
+        for (int r = 0; r < nq; r++) {
+          hh[r].re = 1.0;
+          hh[r].im = 0.0;
+          hh2[r].re = -1.0;
+          hh2[r].im = 0.0;
+        }
+        if ((rc = fmbh_h2d(dh, hh, sizeof(fmb_c64) * nq, stream))) goto done;
+        if ((rc = fmbh_h2d(dh2, hh2, sizeof(fmb_c64) * nq, stream))) goto done;
+        /* r_p = 1 * b_p + (-1) * A x_p */
+        if ((rc = fmbh_zaxpby(dh2, dw + o, dh, dr + o, nq, N, stream))) goto done;
+        if ((rc = fmbh_znorms(dr + o, nq, N, dn, stream))) goto done;
+        if ((rc = fmbh_d2h(hn, dn, sizeof(double) * nq, stream))) goto done;
+        int pass = 1;
+        for (int r = 0; r < nq; r++) pass &= hn[r] / bn[S.off[p] + r] <= tol;
+        if (!pass) continue;
+        S.live[p] = 0;
+        for (int r = 0; r < nq; r++) rsave[S.off[p] + r] = hn[r];
+        graphs_drop(&S);
+        if ((rc = fmbh_memset(dw + o, 0, sizeof(fmb_c64) * nq * (size_t)N, stream))) goto done;
+      }
       if (all_done) {
         j++;
         break;
       }
     }
     /* x += V y, y from the triangular solve (inactive rhs get y = 0) */
-    for (int r = 0; r < nrhs; r++) {
-      for (int q = j - 1; q >= 0; q--) {
-        zc s = g[(size_t)r * (m + 1) + q];
-        for (int c2 = q + 1; c2 < j; c2++) s -= HH(r, q, c2) * yv[(size_t)r * (m + 1) + c2];
-        zc d = HH(r, q, q);
-        yv[(size_t)r * (m + 1) + q] = (active[r] && cabs(d) > 0) ? s / d : 0.0;
-      }
-      for (int q = 0; q < j; q++)
-        hh[(size_t)q * nrhs + r] = fromz(yv[(size_t)r * (m + 1) + q]);
-    }
-    if ((rc = fmbh_h2d(dy, hh, sizeof(fmb_c64) * j * nrhs, stream))) goto done;
-    if ((rc = fmbh_zgemv_add(dV, j, dy, x, nrhs, N, stream))) goto done;
+    if ((rc = update_x(Hh, g, yv, hh, m, nrhs, j, active, dV, dy, x, N, stream))) goto done;
   }
   rc = conv ? FMB_OK : fmb_set_error(FMB_ERR_NOCONV,
                                      "GMRES: tolerance not reached in %d steps",
                                      iters);
-  if (info) {
-    memset(info, 0, sizeof(*info));
-    info->iters = iters;
-    info->matvecs_fp64 = mv64;
-    info->matvecs_work = mvw;
-    info->restarts = restarts;
-    for (int r = 0; r < nrhs && r < 64; r++) info->relres[r] = rn[r] / bn[r];
-  }
+  if (info)
+    for (int p = 0; p < np; p++) {
+      fmb_gmres_info *ip = info + p;
+      memset(ip, 0, sizeof(*ip));
+      ip->iters = iters;
+      ip->matvecs_fp64 = mv64;
+      ip->matvecs_work = mvw;
+      ip->restarts = restarts;
+      for (int r = 0; r < nrs[p] && r < 64; r++)
+        ip->relres[r] = rn[S.off[p] + r] / bn[S.off[p] + r];
+    }
 #undef HH
 done:
-  if (info) info->seconds = now_s() - t0;
+  if (info)
+    for (int p = 0; p < np; p++) info[p].seconds = now_s() - t0;
+  fmbh_sync(stream);
+  sys_free(&S);
   fmbh_free(dV);
   fmbh_free(dw);
   fmbh_free(dr);
@@ -220,18 +435,49 @@ done:
   free(bn);
   free(rn);
   free(hn);
+  free(rsave);
   free(active);
+  free(sel);
   return rc;
 }
 
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream) {
-  return gmres_run(pl, 0, b, x, nrhs, restart, tol, maxit, info, stream);
+  return gmres_run(&pl, &nrhs, 1, 0, 0, b, x, restart, tol, maxit, info, stream);
 }
 
 int fmb_gmres_transpose(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
                         int restart, double tol, int maxit, fmb_gmres_info *info,
                         void *stream) {
-  return gmres_run(pl, 1, b, x, nrhs, restart, tol, maxit, info, stream);
+  return gmres_run(&pl, &nrhs, 1, 1, 0, b, x, restart, tol, maxit, info, stream);
+}
+
+int fmb_gmres_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
+                    const fmb_c64 *b, fmb_c64 *x, int restart, double tol, int maxit,
+                    fmb_gmres_info *info, void *stream) {
+  return gmres_run(plans, nrhs, nplans, 0, graph, b, x, restart, tol, maxit, info, stream);
+}
+
+int fmb_matvec_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
+                     int prec, const fmb_c64 *x, fmb_c64 *y, int reps, void *stream) {
+  if (!x || !y || reps < 1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_matvec_batch: invalid arguments");
+  sys_t S;
+  int rc = sys_init(&S, plans, nrhs, nplans, 0, graph);
+  fmb_c64 *ys = NULL;
+  const size_t nv = (size_t)(rc ? 0 : S.off[S.np]) * S.N;
+  if (!rc && graph) {
+    if (!(rc = fmbh_malloc((void **)&ys, sizeof(fmb_c64) * nv))) rc = sys_graph_init(&S, ys);
+  }
+  for (int r = 0; r < reps && !rc; r++) {
+    fmb_c64 *out = y + (size_t)r * nv;
+    rc = apply(&S, prec, x, graph ? ys : out, stream);
+    if (!rc && graph) rc = fmbh_d2d(out, ys, sizeof(fmb_c64) * nv, stream);
+    if (!rc) rc = fmbh_sync(stream);
+  }
+  fmbh_sync(stream);
+  sys_free(&S);
+  fmbh_free(ys);
+  return rc;
 }

@@ -133,6 +133,7 @@ static void opset_free_freq(fmb_opset *o) {
   fmbh_free(o->d_gws);
   o->d_P2M = o->d_L2P = o->d_near = o->d_xs = o->d_ys = o->d_gws = NULL;
   o->gws_grid = 0;
+  o->gws_bytes = 0;
 }
 
 static void opset_free_rot(fmb_opset *o) {

@@ -14,6 +14,7 @@
  * and is updated in place; row groups otherwise) and assigned to the
  * workgroup's waves by longest-processing-time first. */
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -305,6 +306,199 @@ int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
   return rc;
 }
 
+/* global workspace of one opset: the largest GWS batch among its
+ * translations (at least min_per_wg bytes per workgroup) */
+static int size_gws(fmb_plan *pl, int op, size_t min_per_wg) {
+  fmb_opset *o = &pl->ops[op];
+  size_t need = min_per_wg;
+  for (int kind = 0; kind < 3; kind++)
+    for (int l = 2; l <= pl->tree->lmax; l++) {
+      const fmb_rcr_tab *tb = &pl->rcr[kind][l];
+      if (tb->mode[op] != FMBH_RCR_GWS) continue;
+      size_t b = fmbh_rcr_gws_bytes(tb->P, o->prec, tb->nt[op]);
+      if (b > need) need = b;
+    }
+  need *= RCR_GWS_GRID;
+  if (need == o->gws_bytes) return FMB_OK;
+  fmbh_free(o->d_gws);
+  o->d_gws = NULL;
+  pl->bytes -= o->gws_bytes;
+  o->gws_bytes = 0;
+  o->gws_grid = 0;
+  if (!need) return FMB_OK;
+  int rc = fmbh_malloc(&o->d_gws, need);
+  if (rc) return rc;
+  o->gws_bytes = need;
+  o->gws_grid = RCR_GWS_GRID;
+  pl->bytes += need;
+  return FMB_OK;
+}
+
+/* ---- autotuning of the tile shapes ----
+ * Each translation (kind, level) of each opset is timed, on the plan's own
+ * box pairs and buffers at nrhs_max right-hand sides, in every valid shape:
+ * LDS mode with 8 or 16 vectors per batch (when the batch fits the 160 KB of
+ * LDS), and global-workspace mode with 8 or 16 vectors and row groups of
+ * 2, 3, 4 or 6 tiles.  The fastest shape (one warm-up, then one timed run,
+ * HIP events on the plan's stream) replaces the fixed rule of
+ * fmbh_rcr_shape. */
+typedef struct {
+  int mode, nt, mt;
+} shape_t;
+
+static int shape_candidates(int P, int prec, shape_t *c) {
+  int n = 0, mt = fmbh_rcr_mt_lds(P);
+  static const int mts[4] = {2, 3, 4, 6};
+  for (int nt = 1; nt <= 2; nt++)
+    if (mt) {
+      shape_t v = {FMBH_RCR_LDS, nt, mt};
+      c[n++] = v;
+    }
+  for (int nt = 1; nt <= 2; nt++)
+    for (int i = 0; i < 4; i++) {
+      shape_t v = {FMBH_RCR_GWS, nt, mts[i]};
+      c[n++] = v;
+    }
+  return n;
+}
+
+static int time_run(fmb_plan *pl, int op, int kind, int l, void *stream, float *ms) {
+  void *e0 = NULL, *e1 = NULL;
+  int rc = fmbh_event_create(&e0);
+  if (!rc) rc = fmbh_event_create(&e1);
+  if (!rc) rc = fmbx_rcr_run(pl, op, kind, l, pl->nrhs_max, stream);
+  if (!rc) rc = fmbh_event_record(e0, stream);
+  if (!rc) rc = fmbx_rcr_run(pl, op, kind, l, pl->nrhs_max, stream);
+  if (!rc) rc = fmbh_event_record(e1, stream);
+  if (!rc) rc = fmbh_sync(stream);
+  if (!rc) rc = fmbh_event_elapsed(e0, e1, ms);
+  fmbh_event_destroy(e0);
+  fmbh_event_destroy(e1);
+  return rc;
+}
+
+/* Shapes already tuned in this process, keyed by what the timing depends
+ * on: precision, kind, orders, chunk count and right-hand sides (a sweep
+ * that revisits a level's order reuses its result). */
+typedef struct {
+  int prec, kind, pin, pout, nchunks, nrhs;
+  shape_t shape;
+  float ms;
+} tuned_t;
+#define TUNE_CACHE 256
+static tuned_t tune_cache[TUNE_CACHE];
+static int tune_cached;
+static pthread_mutex_t tune_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static int cache_find(const tuned_t *key, tuned_t *out) {
+  int hit = 0;
+  pthread_mutex_lock(&tune_lock);
+  for (int i = 0; i < tune_cached && !hit; i++) {
+    const tuned_t *c = &tune_cache[i];
+    if (c->prec == key->prec && c->kind == key->kind && c->pin == key->pin &&
+        c->pout == key->pout && c->nchunks == key->nchunks && c->nrhs == key->nrhs) {
+      *out = *c;
+      hit = 1;
+    }
+  }
+  pthread_mutex_unlock(&tune_lock);
+  return hit;
+}
+
+static void cache_put(const tuned_t *v) {
+  pthread_mutex_lock(&tune_lock);
+  tune_cache[tune_cached < TUNE_CACHE ? tune_cached++ : TUNE_CACHE - 1] = *v;
+  pthread_mutex_unlock(&tune_lock);
+}
+
+static int autotune(fmb_plan *pl, int op, void *stream) {
+  const fmb_tree *t = pl->tree;
+  fmb_opset *o = &pl->ops[op];
+  int L = t->lmax, rc = FMB_OK;
+  /* defined contents for the timing runs */
+  for (int l = 2; l <= L && !rc; l++) {
+    size_t bytes = (size_t)t->nbox[l] * pl->nrhs_max * pl->p[l] * pl->p[l] * o->csz;
+    rc = fmbh_memset(o->d_M[l], 0, bytes, stream);
+    if (!rc) rc = fmbh_memset(o->d_L[l], 0, bytes, stream);
+  }
+  int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
+  rot_offsets(pl->Pmax, hoff);
+  for (int kind = 0; kind < 3 && !rc; kind++)
+    for (int l = (kind ? 3 : 2); l <= L && !rc; l++) {
+      fmb_rcr_tab *tb = &pl->rcr[kind][l];
+      const fmb_rcr_tab *pairs = kind == 0 ? tb : &pl->rcr[1][l];
+      if (!tb->mode[op] || !pairs->nchunks) continue;
+      int mn = tb->pin < tb->pout ? tb->pin : tb->pout;
+      int *coff = (int *)malloc(sizeof(int) * (mn + 1));
+      coax_offsets(tb->pin, tb->pout, coff);
+      tuned_t key = {o->prec, kind, tb->pin, tb->pout, pairs->nchunks, pl->nrhs_max,
+                     {0, 0, 0}, 0.0f}, hit;
+      shape_t c[12];
+      int nc;
+      float best = 0.0f;
+      if (cache_find(&key, &hit)) {
+        /* only the cached shape; timed once more for the report */
+        c[0] = hit.shape;
+        nc = 1;
+        best = 1e30f;
+      } else {
+        rc = time_run(pl, op, kind, l, stream, &best);
+        nc = shape_candidates(tb->P, o->prec, c);
+      }
+      for (int i = 0; i < nc && !rc; i++) {
+        if (c[i].mode == tb->mode[op] && c[i].nt == tb->nt[op] && c[i].mt == tb->mt[op])
+          continue;
+        int *sched = NULL, slen = 0;
+        if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, hoff, coff,
+                              &sched, &slen, stream)))
+          break;
+        if (c[i].mode == FMBH_RCR_LDS) {
+          size_t b = fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
+          if (!b || b > 160 * 1024) {
+            fmbh_free(sched);
+            continue;
+          }
+        } else if ((rc = size_gws(pl, op, fmbh_rcr_gws_bytes(tb->P, o->prec, c[i].nt)))) {
+          fmbh_free(sched);
+          break;
+        }
+        shape_t prev = {tb->mode[op], tb->nt[op], tb->mt[op]};
+        int *psched = tb->d_sched[op], plen = tb->sched_len[op];
+        tb->mode[op] = c[i].mode;
+        tb->nt[op] = c[i].nt;
+        tb->mt[op] = c[i].mt;
+        tb->d_sched[op] = sched;
+        tb->sched_len[op] = slen;
+        float ms;
+        rc = time_run(pl, op, kind, l, stream, &ms);
+        if (!rc && ms < best) {
+          best = ms;
+          fmbh_free(psched);
+        } else {
+          tb->mode[op] = prev.mode;
+          tb->nt[op] = prev.nt;
+          tb->mt[op] = prev.mt;
+          tb->d_sched[op] = psched;
+          tb->sched_len[op] = plen;
+          fmbh_free(sched);
+        }
+      }
+      if (!rc && best >= 1e30f) rc = time_run(pl, op, kind, l, stream, &best);
+      tb->tuned_ms[op] = best;
+      if (!rc) {
+        key.shape.mode = tb->mode[op];
+        key.shape.nt = tb->nt[op];
+        key.shape.mt = tb->mt[op];
+        key.ms = best;
+        if (!cache_find(&key, &hit)) cache_put(&key);
+      }
+      free(coff);
+    }
+  free(hoff);
+  if (!rc) rc = size_gws(pl, op, 0);
+  return rc;
+}
+
 int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
   const fmb_tree *t = pl->tree;
   int L = t->lmax, rc = FMB_OK;
@@ -339,21 +533,11 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     free(hoff);
     if (rc) return rc;
   }
-  /* global workspace: the largest GWS batch of each opset */
-  for (int op = 0; op < pl->nops; op++) {
-    size_t need = 0;
-    for (int kind = 0; kind < 3; kind++)
-      for (int l = 2; l <= L; l++) {
-        const fmb_rcr_tab *tb = &pl->rcr[kind][l];
-        if (tb->mode[op] != FMBH_RCR_GWS) continue;
-        size_t b = fmbh_rcr_gws_bytes(tb->P, pl->ops[op].prec, tb->nt[op]);
-        if (b > need) need = b;
-      }
-    if (!need) continue;
-    pl->ops[op].gws_grid = RCR_GWS_GRID;
-    if ((rc = fmbh_malloc(&pl->ops[op].d_gws, need * RCR_GWS_GRID))) return rc;
-    pl->bytes += need * RCR_GWS_GRID;
-  }
+  for (int op = 0; op < pl->nops; op++)
+    if ((rc = size_gws(pl, op, 0))) return rc;
+  if (pl->prm.autotune && pl->prm.rcr_path != FMB_RCR_VALU)
+    for (int op = 0; op < pl->nops; op++)
+      if ((rc = autotune(pl, op, stream))) return rc;
   return FMB_OK;
 }
 
@@ -492,4 +676,18 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
     j.out = o->d_L[l - 1];
   }
   return fmbh_rcr(&j, o->prec, nrhs, stream);
+}
+
+int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
+                        int *nt, int *mt, double *ms) {
+  if (!pl || kind < 0 || kind > 2 || level < 2 || level > pl->tree->lmax ||
+      (kind && level < 3))
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_tile_shape: invalid arguments");
+  int op = (prec == FMB_MIXED && pl->nops == 2) ? 1 : 0;
+  const fmb_rcr_tab *tb = &pl->rcr[kind][level];
+  if (mode) *mode = tb->mode[op];
+  if (nt) *nt = tb->nt[op];
+  if (mt) *mt = tb->mt[op];
+  if (ms) *ms = tb->tuned_ms[op];
+  return FMB_OK;
 }

@@ -154,6 +154,9 @@ typedef struct fmb_plan_params {
                                 rotation tables are built for its orders
                                 once (initOctree's maxWavNum,
                                 translation.cu:2992); 0: k                  */
+  int autotune;              /* 1: time every valid tile shape of every
+                                matrix-core translation at plan creation
+                                and keep the fastest (fmb_plan_tile_shape)  */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);
@@ -178,6 +181,13 @@ int fmb_plan_orders(const fmb_plan *pl, int *p16);
  * fmb_plan_create at k with the same parameters.  On failure the plan holds
  * no operator: only fmb_plan_retune and fmb_plan_destroy accept it. */
 int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream);
+/* Tile shape of the matrix-core translation kind (0 M2L at level, 1 M2M
+ * level -> level-1, 2 L2L level-1 -> level) in the pass of precision prec:
+ * mode (1: the batch in LDS, 2: in a global workspace, 0: VALU kernel), 16-column
+ * tiles per batch nt (8 nt vectors), row tiles per unit mt, and with
+ * autotune the timed milliseconds of one launch (else 0). */
+int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
+                        int *nt, int *mt, double *ms);
 /* Wavenumber of the operator the plan holds (0 after a failed retune). */
 int fmb_plan_wavenumber(const fmb_plan *pl, double *k);
 /* Device bytes held by the plan. */
@@ -238,6 +248,28 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
  * alias.  The reference has no solver. */
 int fmb_sweep_guess(const fmb_plan *pl, double k_from, const fmb_c64 *x_from,
                     const double *dirs, int nrhs, fmb_c64 *x0, void *stream);
+/* GMRES over several plans at once (the reference has none): plans[p]
+ * (one frequency each, any tree depth and orders; all on meshes with the
+ * same element count) owns nrhs[p] right-hand sides; b and x are device
+ * vectors [sum nrhs][N] with plan 0's columns first.  Every Arnoldi step
+ * applies all plans in lock step (FP64 Arnoldi and true-residual
+ * refinement as fmb_gmres, with the work precision shared by the plans).
+ * graph = 1 captures the plans' FMM passes once per precision into one
+ * hipGraph with one branch per plan (run concurrently, replayed every
+ * step); a plan whose right-hand sides have all converged leaves the graph
+ * at the next restart.  info: nplans reports (may be NULL).  A plan must not
+ * be retuned or destroyed during the call. */
+int fmb_gmres_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
+                    const fmb_c64 *b, fmb_c64 *x, int restart, double tol, int maxit,
+                    fmb_gmres_info *info, void *stream);
+
+/* y = A x for every plan of a batch (plans and columns as fmb_gmres_batch),
+ * eagerly (graph = 0) or by replaying one captured hipGraph reps times (the
+ * output of replay r in y + r * sum(nrhs) * N; eager: reps results of
+ * repeated passes).  The reference has none. */
+int fmb_matvec_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
+                     int prec, const fmb_c64 *x, fmb_c64 *y, int reps, void *stream);
+
 /* y = A^T x (plain transpose, no conjugation) of the operator fmb_matvec
  * applies: the FMM pass run backwards with every operator transposed (L2P^T,
  * L2L^T, M2L^T, M2M^T, P2M^T; near-field blocks read by column).  parts as

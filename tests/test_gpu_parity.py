@@ -218,3 +218,34 @@ def test_field_points_match_oracle(sphere_l3, robin):
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
     print("field robin=%s rel err %.2e" % (robin, err), flush=True)
     assert err < 1e-12
+
+
+def test_autotuned_shapes_give_the_same_operator():
+    """Tile shapes chosen by timing (autotune) against the fixed rule on the
+    5,120-triangle sphere with orders that make the rule pick both LDS and
+    global-workspace batches: the same operator (FP64 pass 1e-13, FP32 pass
+    1e-5 relative; summation order only), every shape reported with its
+    time, and a second plan with the same shapes served from the cache."""
+    m = F.Mesh.icosphere(4, 0.1)
+    k = wavenumber(4000.0)
+    t = F.Tree(m, depth=5)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=3, p_rule=F.PRULE_EXPLICIT,
+              p_levels=[0, 0, 62, 44, 30, 20])
+    ref = F.Plan(t, k, **kw)
+    tuned = F.Plan(t, k, autotune=1, **kw)
+    n = m.arrays()[1].shape[0]
+    rng = np.random.default_rng(31)
+    x = torch.tensor(rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n)), device="cuda")
+    for prec, tol in ((F.FMB_FP64, 1e-13), (F.FMB_MIXED, 1e-5)):
+        a = ref.matvec(x, prec=prec).cpu().numpy()
+        b = tuned.matvec(x, prec=prec).cpu().numpy()
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < tol
+        for kind in range(3):
+            for l in range(2 if kind == 0 else 3, 6):
+                mode, nt, mt, ms = tuned.tile_shape(kind, l, prec)
+                assert mode in (1, 2) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
+                assert ref.tile_shape(kind, l, prec)[3] == 0.0
+    again = F.Plan(t, k, autotune=1, **kw)
+    for kind in range(3):
+        for l in range(2 if kind == 0 else 3, 6):
+            assert again.tile_shape(kind, l, F.FMB_MIXED)[:3] == tuned.tile_shape(kind, l, F.FMB_MIXED)[:3]
