@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="fixed tile-shape rule instead of timing the shapes at plan creation")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the FMM-BEM path here is single-GPU "
@@ -132,11 +134,13 @@ def main():
     mesh = F.Mesh.icosphere(w["level"], w["radius"])
     V, Fc = mesh.arrays()
     tree = F.Tree(mesh, depth=w["depth"])
-    plan = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=w["nrhs"])
+    plan = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=w["nrhs"],
+                  autotune=0 if a.no_autotune else 1)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
-    log("setup %.2fs orders %s mem %.2f GB" % (setup_s, plan.orders()[2:w["depth"] + 1],
-                                              plan.memory() / 1e9))
+    log("setup %.2fs (autotune %s) orders %s mem %.2f GB"
+        % (setup_s, "off" if a.no_autotune else "on", plan.orders()[2:w["depth"] + 1],
+           plan.memory() / 1e9))
     n = Fc.shape[0]
     nrhs = w["nrhs"]
     # one elementwise kernel of known traffic (reads 256 MiB, writes 256 MiB):
@@ -237,6 +241,9 @@ def main():
                      "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
                      "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},
         "cpu_baseline": cpu,
+        "tile_shapes": {
+            name: {str(l): list(plan.tile_shape(0, l, prec)[:3]) for l in range(2, w["depth"] + 1)}
+            for name, prec in (("m2l_fp32", F.FMB_MIXED), ("m2l_fp64", F.FMB_FP64))},
     }
     if solve:
         out["solve"] = {kk: {q: v for q, v in s.items()} for kk, s in solve.items()}

@@ -81,6 +81,7 @@ int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 /* matrix-core RCR translations (rcr_mfma.hip) */
 #define FMBH_RCR_LDS 1 /* batch in LDS, blocks owned by one wave          */
 #define FMBH_RCR_GWS 2 /* batch in a global workspace, row-group units    */
+#define FMBH_RCR_LDSP 3 /* as LDS, a persistent grid looping over batches */
 typedef struct fmbh_rcr_job {
   int mode, nt, mt;
   const void *chunks;

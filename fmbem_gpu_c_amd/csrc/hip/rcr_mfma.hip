@@ -389,6 +389,28 @@ __global__ void __launch_bounds__(kThreads)
   rcr_batch<T, NT, MT, false>(a, ch, u0, X, X, ph, sched);
 }
 
+// LDS mode on a persistent grid (one workgroup per free slot of the GPU):
+// each workgroup loads the schedule once and loops over the items chunk * nb
+// + batch, so workgroups are not dispatched (and set up) per batch.
+template <typename T, int NT, int MT>
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) k_rcr_ldsp(RcrArgs a) {
+  constexpr int V = 8 * NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int Pm = a.pin > a.pout ? a.pin : a.pout;
+  T *X = (T *)smem;
+  cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * Row<NT>::stride);
+  int *sched = (int *)(ph + Pm + (Pm & 1));
+  for (int i = threadIdx.x; i < a.sched_len; i += kThreads) sched[i] = a.sched[i];
+  const long long items = (long long)a.nchunks * a.nb;
+  for (long long it = blockIdx.x; it < items; it += gridDim.x) {
+    const int c = (int)(it / a.nb), b = (int)(it % a.nb);
+    const int4 ch = a.chunks[c];
+    if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
+    rcr_batch<T, NT, MT, false>(a, ch, b * V, X, X, ph, sched);
+  }
+}
+
 // GWS mode: persistent grid over items chunk * nb + batch.
 template <typename T, int NT, int MT>
 __global__ void __launch_bounds__(kThreads) k_rcr_gws(RcrArgs a) {
@@ -504,6 +526,19 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
     int rc = set_lds(kern, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(kern, dim3(a.nchunks, a.nb), dim3(kThreads), lds, st, a);
+  } else if (j->mode == FMBH_RCR_LDSP) {
+    const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
+    auto kern = k_rcr_ldsp<T, NT, MT>;
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    int per_cu = 0, ncu = 0, dev = 0;
+    FMB_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, lds));
+    FMB_HIP_CHECK(hipGetDevice(&dev));
+    FMB_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (per_cu < 1) per_cu = 1;
+    long long items = (long long)a.nchunks * a.nb, cap = (long long)per_cu * ncu;
+    int grid = (int)(items < cap ? items : cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, st, a);
   } else {
     const size_t lds = small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_gws<T, NT, MT>;

@@ -236,14 +236,23 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
   active = (int *)malloc(sizeof(int) * nrhs);
   sel = (int *)malloc(sizeof(int) * nrhs);
 #define HH(r, i, j) Hh[((size_t)(r) * m + (j)) * (m + 1) + (i)]
+  if ((rc = fmbh_znorms(x, nrhs, N, dn, stream))) goto done;
+  if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
+  int x_zero = 1;
+  for (int r = 0; r < nrhs; r++) x_zero &= bn[r] == 0.0;
   if ((rc = fmbh_znorms(b, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
   for (int r = 0; r < nrhs; r++)
     if (bn[r] == 0.0) bn[r] = 1.0;
   while (1) {
-    /* true residual r = b - A x in FP64 */
-    if ((rc = apply(&S, FMB_FP64, x, dw, stream))) goto done;
-    mv64++;
+    /* true residual r = b - A x in FP64 (A x = 0 for a zero first guess) */
+    if (x_zero) {
+      if ((rc = fmbh_memset(dw, 0, vbytes, stream))) goto done;
+      x_zero = 0;
+    } else {
+      if ((rc = apply(&S, FMB_FP64, x, dw, stream))) goto done;
+      mv64++;
+    }
     {
       fmb_c64 *one = hh, *mone = hh2;
       for (int r = 0; r < nrhs; r++) {

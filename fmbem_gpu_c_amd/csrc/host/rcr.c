@@ -351,8 +351,9 @@ static int shape_candidates(int P, int prec, shape_t *c) {
   static const int mts[4] = {2, 3, 4, 6};
   for (int nt = 1; nt <= 2; nt++)
     if (mt) {
-      shape_t v = {FMBH_RCR_LDS, nt, mt};
+      shape_t v = {FMBH_RCR_LDS, nt, mt}, w = {FMBH_RCR_LDSP, nt, mt};
       c[n++] = v;
+      c[n++] = w;
     }
   for (int nt = 1; nt <= 2; nt++)
     for (int i = 0; i < 4; i++) {
@@ -433,7 +434,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
       coax_offsets(tb->pin, tb->pout, coff);
       tuned_t key = {o->prec, kind, tb->pin, tb->pout, pairs->nchunks, pl->nrhs_max,
                      {0, 0, 0}, 0.0f}, hit;
-      shape_t c[12];
+      shape_t c[16];
       int nc;
       float best = 0.0f;
       if (cache_find(&key, &hit)) {
@@ -452,7 +453,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, hoff, coff,
                               &sched, &slen, stream)))
           break;
-        if (c[i].mode == FMBH_RCR_LDS) {
+        if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP) {
           size_t b = fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
           if (!b || b > 160 * 1024) {
             fmbh_free(sched);

@@ -183,7 +183,8 @@ int fmb_plan_orders(const fmb_plan *pl, int *p16);
 int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream);
 /* Tile shape of the matrix-core translation kind (0 M2L at level, 1 M2M
  * level -> level-1, 2 L2L level-1 -> level) in the pass of precision prec:
- * mode (1: the batch in LDS, 2: in a global workspace, 0: VALU kernel), 16-column
+ * mode (1: the batch in LDS, 2: in a global workspace, 3: in LDS on a
+ * persistent grid, 0: VALU kernel), 16-column
  * tiles per batch nt (8 nt vectors), row tiles per unit mt, and with
  * autotune the timed milliseconds of one launch (else 0). */
 int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,

@@ -243,9 +243,38 @@ def test_autotuned_shapes_give_the_same_operator():
         for kind in range(3):
             for l in range(2 if kind == 0 else 3, 6):
                 mode, nt, mt, ms = tuned.tile_shape(kind, l, prec)
-                assert mode in (1, 2) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
+                assert mode in (1, 2, 3) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
                 assert ref.tile_shape(kind, l, prec)[3] == 0.0
     again = F.Plan(t, k, autotune=1, **kw)
     for kind in range(3):
         for l in range(2 if kind == 0 else 3, 6):
             assert again.tile_shape(kind, l, F.FMB_MIXED)[:3] == tuned.tile_shape(kind, l, F.FMB_MIXED)[:3]
+
+
+def test_config1_depth3_order8_vs_oracle_and_mie(sphere_l3):
+    """The plain-method configuration as specified: 1,280 flat triangles,
+    r = 0.1 m, 1 kHz, plane wave along +z, Burton-Miller (alpha = i/k),
+    7-point rule with analytic singular self terms, octree depth 3 with
+    order 8 at the leaves (16 at level 2, truncNum's), GMRES(30), FP64.
+    The solution against the oracle's FMM solve of the same operator
+    (1e-8 at tolerance 1e-10) and the surface pressure against the Mie
+    series at 64 surface points (2%)."""
+    m, V, Fc = sphere_l3
+    n = Fc.shape[0]
+    k = wavenumber(1000.0)
+    t = F.Tree(m, depth=3)
+    pl = F.Plan(t, k, p_rule=F.PRULE_EXPLICIT, p_levels=[0, 0, 16, 8])
+    assert pl.orders()[2:4].tolist() == [16, 8]
+    b = pl.rhs_plane_wave([0.0, 0.0, 1.0])
+    x, info = pl.gmres(b, restart=30, tol=1e-10)
+    u = x[0].cpu().numpy()
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(depth=3, plev={2: 16, 3: 8})
+    xo, it, rr, rc = ob.gmres(ob.rhs_plane([0, 0, 1.0]), use_fmm=True, m=30, tol=1e-10)
+    assert rc == 0
+    assert np.linalg.norm(u - xo) / np.linalg.norm(xo) < 1e-8
+    idx = np.linspace(0, n - 1, 64).astype(int)
+    c, _, _ = ob.centroids()
+    ct = c[idx, 2] / np.linalg.norm(c[idx], axis=1)
+    mie = O.mie_rigid(k * 0.1, ct)
+    assert np.linalg.norm(u[idx] - mie) / np.linalg.norm(mie) < 0.02
