@@ -49,6 +49,7 @@ class PlanParams(ctypes.Structure):
         ("rcr_path", ctypes.c_int),
         ("k_max", ctypes.c_double),
         ("autotune", ctypes.c_int),
+        ("serial_levels", ctypes.c_int),
     ]
 
 

@@ -64,8 +64,9 @@ typedef struct fmb_opset {
   void *d_near; /* near blocks, complex                                     */
   void *d_M[FMB_MAXLEV], *d_L[FMB_MAXLEV]; /* [nbox][nrhs][p^2] complex      */
   void *d_xs, *d_ys; /* sorted-order vectors [nrhs][N] complex                */
-  void *d_gws;       /* global workspace of the GWS-mode translations       */
-  size_t gws_bytes;
+  void *d_gws;       /* global workspace of the GWS-mode translations: one
+                        slot per stream the plan launches them on         */
+  size_t gws_bytes, gws_slot;
   void *d_Hpad;      /* zero-padded rotation blocks [nbeta][hpad_stride]    */
   void *d_phase[3];  /* e^{i m phi}, m < Pmax: M2L offsets, M2M, L2L octants */
   void *d_phase_c[3]; /* e^{-i m phi}: the same buckets, transposed pass    */
@@ -133,6 +134,10 @@ struct fmb_plan {
   size_t bytes;         /* device bytes of the rotation + frequency tiers  */
   size_t bytes_rot, bytes_freq;
   int ready;            /* frequency tier built (cleared by a failed retune) */
+  /* a second stream for the coarse M2L levels and the L2L chain, and the
+   * events ordering it against the fine M2L levels (NULL: one stream) */
+  void *aux, *ev_up, *ev_aux, *ev_m2l[FMB_MAXLEV];
+  int capturing;        /* set while a batch captures this plan's pass      */
   void *stream;
 };
 

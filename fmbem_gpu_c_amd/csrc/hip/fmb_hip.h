@@ -27,6 +27,7 @@ int fmbh_event_create(void **ev);
 int fmbh_event_record(void *ev, void *stream);
 int fmbh_event_elapsed(void *ev0, void *ev1, float *ms);
 int fmbh_event_destroy(void *ev);
+int fmbh_event_create_sync(void **ev);
 int fmbh_stream_create(void **s);
 int fmbh_stream_destroy(void *s);
 int fmbh_stream_wait(void *stream, void *ev);

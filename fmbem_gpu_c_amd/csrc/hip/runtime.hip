@@ -149,3 +149,11 @@ extern "C" int fmbh_graph_destroy(void *exec) {
   if (exec) FMB_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)exec));
   return FMB_OK;
 }
+
+/* fork/join events (no timestamps) */
+extern "C" int fmbh_event_create_sync(void **ev) {
+  hipEvent_t e;
+  FMB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *ev = e;
+  return FMB_OK;
+}

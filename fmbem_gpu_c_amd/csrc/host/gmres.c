@@ -75,10 +75,14 @@ static int graph_capture(sys_t *S, int prec, void **exec) {
     fmb_plan *pl = S->pl[p];
     int tm = pl->timing;
     pl->timing = 0;
+    /* the plan's own second stream is not forked inside the capture: that
+     * nested fork ended the process in the HIP runtime on this stack */
+    pl->capturing = 1;
     size_t o = (size_t)S->off[p] * S->N;
     rc = fmbh_stream_wait(S->bs[p], S->ev_fork);
     if (!rc) rc = plan_apply(pl, S->trans, prec, S->xs + o, S->ys + o, S->nr[p], S->bs[p]);
     pl->timing = tm;
+    pl->capturing = 0;
     if (!rc) rc = fmbh_event_record(S->ev_join[p], S->bs[p]);
     if (!rc) rc = fmbh_stream_wait(S->cs, S->ev_join[p]);
   }

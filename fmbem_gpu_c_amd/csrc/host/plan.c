@@ -134,6 +134,7 @@ static void opset_free_freq(fmb_opset *o) {
   o->d_P2M = o->d_L2P = o->d_near = o->d_xs = o->d_ys = o->d_gws = NULL;
   o->gws_grid = 0;
   o->gws_bytes = 0;
+  o->gws_slot = 0;
 }
 
 static void opset_free_rot(fmb_opset *o) {
@@ -189,6 +190,11 @@ void fmb_plan_destroy(fmb_plan *pl) {
   fmbh_free(pl->d_near_off);
   fmbh_free(pl->d_near_ncol);
   fmbh_free(pl->d_work64);
+  if (pl->aux) fmbh_sync(pl->aux);
+  fmbh_stream_destroy(pl->aux);
+  fmbh_event_destroy(pl->ev_up);
+  fmbh_event_destroy(pl->ev_aux);
+  for (int l = 0; l < FMB_MAXLEV; l++) fmbh_event_destroy(pl->ev_m2l[l]);
   free(pl);
 }
 
@@ -306,6 +312,15 @@ static int plan_geometry(fmb_plan *pl, void *stream) {
   free(gw);
   free(e);
   if (rc) return rc;
+  /* the matrix-core L2L adds with atomics (the VALU one does not, so it
+   * cannot share L_l with that level's M2L) */
+  if (!prm->serial_levels && L >= 5 && prm->rcr_path != FMB_RCR_VALU) {
+    if ((rc = fmbh_stream_create(&pl->aux))) return rc;
+    if ((rc = fmbh_event_create_sync(&pl->ev_up))) return rc;
+    if ((rc = fmbh_event_create_sync(&pl->ev_aux))) return rc;
+    for (int l = 2; l <= L; l++)
+      if ((rc = fmbh_event_create_sync(&pl->ev_m2l[l]))) return rc;
+  }
   return fmbh_near_layout(pl, stream);
 }
 
@@ -575,11 +590,45 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
   for (int l = L; l > 2; l--)
     if ((rc = fmbx_rcr_run(pl, op, 1, l, nrhs, stream))) goto out;
   if (tm) fmbh_event_record(ev[2], stream);
-  for (int l = 2; l <= L; l++) {
-    if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
-    if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, stream))) goto out;
-    if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
-    if (l > 2 && (rc = fmbx_rcr_run(pl, op, 2, l, nrhs, stream))) goto out;
+  /* inside a graph capture the levels stay on the capturing stream (see
+   * fmb_gmres_batch: a second fork there is not used) */
+  if (!pl->aux || pl->capturing) {
+    for (int l = 2; l <= L; l++) {
+      if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
+      if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, stream))) goto out;
+      if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
+      if (l > 2 && (rc = fmbx_rcr_run(pl, op, 2, l, nrhs, stream))) goto out;
+    }
+  } else {
+    /* M2L levels need only the upward pass; L2L into level l reads L_{l-1}
+     * complete (its M2L and L2L), and adds into L_l with atomics beside
+     * that level's M2L.  The caller's stream runs the three finest M2L
+     * levels, coarsest first; the second stream runs the coarser M2L
+     * levels and the L2L chain, waiting for a fine level's M2L before the
+     * L2L that reads it. */
+    const int ls = L - 2;
+    void *aux = pl->aux;
+    if ((rc = fmbh_event_record(pl->ev_up, stream))) goto out;
+    if ((rc = fmbh_stream_wait(aux, pl->ev_up))) goto out;
+    for (int l = ls; l <= L; l++) {
+      if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
+      if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, stream))) goto out;
+      if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
+      if ((rc = fmbh_event_record(pl->ev_m2l[l], stream))) goto out;
+    }
+    for (int l = 2; l <= L; l++) {
+      if (l < ls) {
+        if (tm) fmbh_event_record(ev[4 + 2 * l], aux);
+        if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, aux))) goto out;
+        if (tm) fmbh_event_record(ev[5 + 2 * l], aux);
+      }
+      if (l > 2) {
+        if (l - 1 >= ls && (rc = fmbh_stream_wait(aux, pl->ev_m2l[l - 1]))) goto out;
+        if ((rc = fmbx_rcr_run(pl, op, 2, l, nrhs, aux))) goto out;
+      }
+    }
+    if ((rc = fmbh_event_record(pl->ev_aux, aux))) goto out;
+    if ((rc = fmbh_stream_wait(stream, pl->ev_aux))) goto out;
   }
   if (tm) fmbh_event_record(ev[3], stream);
   if ((rc = fmbh_l2p_near(pl, op, nrhs, parts, stream))) goto out;

@@ -319,6 +319,9 @@ static int size_gws(fmb_plan *pl, int op, size_t min_per_wg) {
       if (b > need) need = b;
     }
   need *= RCR_GWS_GRID;
+  o->gws_slot = need;
+  /* a second slot for the launches on the plan's second stream */
+  if (pl->aux) need *= 2;
   if (need == o->gws_bytes) return FMB_OK;
   fmbh_free(o->d_gws);
   o->d_gws = NULL;
@@ -590,7 +593,7 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.pout = tb->pout;
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
-  j.gws = o->d_gws;
+  j.gws = (pl->aux && stream == pl->aux) ? (char *)o->d_gws + o->gws_slot : o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase[kind];
   j.pstride = pl->Pmax;

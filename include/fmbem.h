@@ -157,6 +157,10 @@ typedef struct fmb_plan_params {
   int autotune;              /* 1: time every valid tile shape of every
                                 matrix-core translation at plan creation
                                 and keep the fastest (fmb_plan_tile_shape)  */
+  int serial_levels;         /* 1: M2L levels one after another on the
+                                caller's stream; 0: the coarse levels and
+                                the L2L chain on a second stream, beside
+                                the three finest M2L levels                 */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);
