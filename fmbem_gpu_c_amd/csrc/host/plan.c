@@ -605,7 +605,7 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
      * that level's M2L.  The caller's stream runs the three finest M2L
      * levels, coarsest first; the second stream runs the coarser M2L
      * levels and the L2L chain, waiting for a fine level's M2L before the
-     * L2L that reads it. */
+     * L2L that reads it, and for the finest M2L before the last L2L. */
     const int ls = L - 2;
     void *aux = pl->aux;
     if ((rc = fmbh_event_record(pl->ev_up, stream))) goto out;
@@ -624,6 +624,8 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
       }
       if (l > 2) {
         if (l - 1 >= ls && (rc = fmbh_stream_wait(aux, pl->ev_m2l[l - 1]))) goto out;
+        /* the finest M2L (the longest launch) runs alone */
+        if (l == L && (rc = fmbh_stream_wait(aux, pl->ev_m2l[L]))) goto out;
         if ((rc = fmbx_rcr_run(pl, op, 2, l, nrhs, aux))) goto out;
       }
     }
