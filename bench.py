@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="fixed tile-shape rule instead of timing the shapes at plan creation")
+    ap.add_argument("--serial-levels", action="store_true",
+                    help="all translations on one stream (per-level times without overlap)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the FMM-BEM path here is single-GPU "
@@ -135,7 +137,8 @@ def main():
     V, Fc = mesh.arrays()
     tree = F.Tree(mesh, depth=w["depth"])
     plan = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=w["nrhs"],
-                  autotune=0 if a.no_autotune else 1)
+                  autotune=0 if a.no_autotune else 1,
+                  serial_levels=1 if a.serial_levels else 0)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     log("setup %.2fs (autotune %s) orders %s mem %.2f GB"

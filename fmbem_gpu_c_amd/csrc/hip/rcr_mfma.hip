@@ -358,15 +358,22 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   __syncthreads();
   rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
   __syncthreads();
-  // add e^{-i m' phi} y into the targets
-  for (int f = tid; f < Po2; f += kThreads) {
-    const int n = isqrt_fix(f), m = f - n * n - n;
+  // add e^{-i m' phi} y into the targets.  Thread e adds real e of each
+  // target vector (part e & 1 of coefficient e >> 1), so that one wave's
+  // atomic adds fill whole cache lines: the adds are performed beyond the
+  // L2, and half-filled lines cost nearly as much as full ones.
+  for (int e = tid; e < 2 * Po2; e += kThreads) {
+    const int f = e >> 1, n = isqrt_fix(f), m = f - n * n - n;
+    const bool im = e & 1;
     cplx<T> w = ph[m < 0 ? -m : m];
     if (m > 0) w.im = -w.im;
 #pragma unroll
     for (int v = 0; v < V; v++)
-      if (vout[v])
-        atomic_add_c(vout[v] + (unsigned)f, cmul(*(const cplx<T> *)(X1 + f * RS + 2 * v), w));
+      if (vout[v]) {
+        const cplx<T> x = *(const cplx<T> *)(X1 + f * RS + 2 * v);
+        const T y = im ? x.re * w.im + x.im * w.re : x.re * w.re - x.im * w.im;
+        unsafeAtomicAdd((T *)vout[v] + (unsigned)e, y);
+      }
   }
 }
 
