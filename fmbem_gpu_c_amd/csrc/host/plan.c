@@ -602,21 +602,22 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
   } else {
     /* M2L levels need only the upward pass; L2L into level l reads L_{l-1}
      * complete (its M2L and L2L), and adds into L_l with atomics beside
-     * that level's M2L.  The caller's stream runs the three finest M2L
-     * levels, coarsest first; the second stream runs the coarser M2L
-     * levels and the L2L chain, waiting for a fine level's M2L before the
-     * L2L that reads it, and for the finest M2L before the last L2L. */
+     * that level's M2L.  The caller's stream runs the M2L levels L-2 and
+     * L-1; the second stream runs the coarser M2L levels and the L2L chain
+     * up to level L-1, waiting for a fine level's M2L before the L2L that
+     * reads it.  The finest M2L (the longest launch) then runs alone on the
+     * caller's stream, followed by the last L2L. */
     const int ls = L - 2;
     void *aux = pl->aux;
     if ((rc = fmbh_event_record(pl->ev_up, stream))) goto out;
     if ((rc = fmbh_stream_wait(aux, pl->ev_up))) goto out;
-    for (int l = ls; l <= L; l++) {
+    for (int l = ls; l < L; l++) {
       if (tm) fmbh_event_record(ev[4 + 2 * l], stream);
       if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, stream))) goto out;
       if (tm) fmbh_event_record(ev[5 + 2 * l], stream);
       if ((rc = fmbh_event_record(pl->ev_m2l[l], stream))) goto out;
     }
-    for (int l = 2; l <= L; l++) {
+    for (int l = 2; l < L; l++) {
       if (l < ls) {
         if (tm) fmbh_event_record(ev[4 + 2 * l], aux);
         if ((rc = fmbx_rcr_run(pl, op, 0, l, nrhs, aux))) goto out;
@@ -624,13 +625,15 @@ int fmb_matvec_parts(fmb_plan *pl, int prec, int parts, const fmb_c64 *x,
       }
       if (l > 2) {
         if (l - 1 >= ls && (rc = fmbh_stream_wait(aux, pl->ev_m2l[l - 1]))) goto out;
-        /* the finest M2L (the longest launch) runs alone */
-        if (l == L && (rc = fmbh_stream_wait(aux, pl->ev_m2l[L]))) goto out;
         if ((rc = fmbx_rcr_run(pl, op, 2, l, nrhs, aux))) goto out;
       }
     }
     if ((rc = fmbh_event_record(pl->ev_aux, aux))) goto out;
     if ((rc = fmbh_stream_wait(stream, pl->ev_aux))) goto out;
+    if (tm) fmbh_event_record(ev[4 + 2 * L], stream);
+    if ((rc = fmbx_rcr_run(pl, op, 0, L, nrhs, stream))) goto out;
+    if (tm) fmbh_event_record(ev[5 + 2 * L], stream);
+    if ((rc = fmbx_rcr_run(pl, op, 2, L, nrhs, stream))) goto out;
   }
   if (tm) fmbh_event_record(ev[3], stream);
   if ((rc = fmbh_l2p_near(pl, op, nrhs, parts, stream))) goto out;
