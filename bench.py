@@ -245,7 +245,7 @@ def main():
                      "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},
         "cpu_baseline": cpu,
         "tile_shapes": {
-            name: {str(l): list(plan.tile_shape(0, l, prec)[:3]) for l in range(2, w["depth"] + 1)}
+            name: {str(l): list(plan.tile_shape(0, l, prec)[:4]) for l in range(2, w["depth"] + 1)}
             for name, prec in (("m2l_fp32", F.FMB_MIXED), ("m2l_fp64", F.FMB_FP64))},
     }
     if solve:

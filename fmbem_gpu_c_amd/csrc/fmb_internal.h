@@ -81,6 +81,7 @@ typedef struct fmb_rcr_tab {
   int nchunks;
   int pin, pout, P;     /* source order, target order, coaxial order       */
   int mode[2], nt[2], mt[2]; /* per opset: 0 = VALU kernel, else FMBH_RCR_* */
+  int nw[2];            /* per opset: waves per workgroup (4 or 8)         */
   int *d_sched[2];      /* per opset: unit schedule                        */
   int sched_len[2];     /* ints in d_sched                                 */
   void *d_Cpad[2];      /* per opset: zero-padded coaxial blocks           */

@@ -84,7 +84,7 @@ int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 #define FMBH_RCR_GWS 2 /* batch in a global workspace, row-group units    */
 #define FMBH_RCR_LDSP 3 /* as LDS, a persistent grid looping over batches */
 typedef struct fmbh_rcr_job {
-  int mode, nt, mt;
+  int mode, nt, mt, nw; /* nw: waves per workgroup, 4 or 8 */
   const void *chunks;
   int nchunks, chunk_pairs;
   const void *info;

@@ -1,0 +1,8 @@
+// rcr_f32_w4.hip -- the FP32 translation kernels with 4 waves per workgroup
+// (rcr_kernels.h), one translation unit per precision and wave count so
+// that they compile in parallel.
+#include "rcr_kernels.h"
+
+int fmbh_rcr_detail::run_f32_w4(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
+  return dispatch_shape<float, 4>(j, nrhs, st);
+}

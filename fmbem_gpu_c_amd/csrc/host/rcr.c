@@ -23,7 +23,7 @@
 #include "host.h"
 
 #define RCR_CHUNK 16
-#define RCR_WAVES 4
+#define RCR_WAVES_MAX 8
 #define RCR_GWS_GRID 512
 
 typedef struct {
@@ -41,26 +41,26 @@ static int by_cost(const void *a, const void *b) {
   return x->code - y->code;
 }
 
-/* units in non-increasing cost order, each to the least-loaded wave;
- * writes the waves' lists back to back as {code, aoff} pairs into out
- * (entry q at out[2q]) and their start entries (RCR_WAVES + 1 values,
+/* units in non-increasing cost order, each to the least-loaded of nw
+ * waves; writes the waves' lists back to back as {code, aoff} pairs into
+ * out (entry q at out[2q]) and their start entries (nw + 1 values,
  * beginning at base) into off */
-static void lpt(unit_t *u, int n, int base, int *off, int *out) {
-  int load[RCR_WAVES] = {0}, cnt[RCR_WAVES] = {0};
+static void lpt(unit_t *u, int n, int nw, int base, int *off, int *out) {
+  int load[RCR_WAVES_MAX] = {0}, cnt[RCR_WAVES_MAX] = {0};
   int *w_of = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   qsort(u, n, sizeof(unit_t), by_cost);
   for (int i = 0; i < n; i++) {
     int w = 0;
-    for (int q = 1; q < RCR_WAVES; q++)
+    for (int q = 1; q < nw; q++)
       if (load[q] < load[w]) w = q;
     load[w] += u[i].cost;
     w_of[i] = w;
     cnt[w]++;
   }
   off[0] = base;
-  for (int w = 0; w < RCR_WAVES; w++) off[w + 1] = off[w] + cnt[w];
-  int fill[RCR_WAVES];
-  for (int w = 0; w < RCR_WAVES; w++) fill[w] = off[w];
+  for (int w = 0; w < nw; w++) off[w + 1] = off[w] + cnt[w];
+  int fill[RCR_WAVES_MAX];
+  for (int w = 0; w < nw; w++) fill[w] = off[w];
   for (int i = 0; i < n; i++) {
     int q = fill[w_of[i]]++;
     out[2 * q] = u[i].code;
@@ -127,26 +127,29 @@ static int coax_units(int pin, int pout, int split, int mt, const int *coff,
   return k;
 }
 
-/* Schedule of one translation shape: a header of three offset tables
- * (forward rotation, coaxial, back rotation), RCR_WAVES + 1 entry indices
+/* Schedule of one translation shape for nw waves: a header of three offset
+ * tables (forward rotation, coaxial, back rotation), nw + 1 entry indices
  * each, giving every wave's units as entries [off[w], off[w+1]); entry q is
- * the int pair {code, operator offset} at [2q, 2q + 1] of the same array. */
-#define RCR_SCHED_HDR 16 /* 3 (RCR_WAVES + 1) ints, rounded up to even */
-static int build_sched(int pin, int pout, int P, int mode, int mt, const int *hoff,
+ * the int pair {code, operator offset} at [2q, 2q + 1] of the same array.
+ * The header is 3 (nw + 1) ints rounded up to even. */
+static int build_sched(int pin, int pout, int P, int mode, int mt, int nw, const int *hoff,
                        const int *coff, int **d_out, int *len, void *stream) {
   int split = mode == FMBH_RCR_GWS, rc;
+  if (nw < 1 || nw > RCR_WAVES_MAX)
+    return fmb_set_error(FMB_ERR_RANGE, "translation schedule for %d waves", nw);
+  const int hdr = (3 * (nw + 1) + 1) & ~1;
   unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + 2 * P + pout));
   int n1 = rot_units(pin, split, mt, hoff, u);
   int nc = coax_units(pin, pout, split, mt, coff, u + n1);
   int n3 = rot_units(pout, split, mt, hoff, u + n1 + nc);
-  int tot = RCR_SCHED_HDR + 2 * (n1 + nc + n3);
+  int tot = hdr + 2 * (n1 + nc + n3);
   int *h = (int *)calloc(tot, sizeof(int));
-  int base = RCR_SCHED_HDR / 2;
-  lpt(u, n1, base, h, h);
+  int base = hdr / 2;
+  lpt(u, n1, nw, base, h, h);
   base += n1;
-  lpt(u + n1, nc, base, h + RCR_WAVES + 1, h);
+  lpt(u + n1, nc, nw, base, h + nw + 1, h);
   base += nc;
-  lpt(u + n1 + nc, n3, base, h + 2 * (RCR_WAVES + 1), h);
+  lpt(u + n1 + nc, n3, nw, base, h + 2 * (nw + 1), h);
   *len = tot;
   rc = fmbh_malloc((void **)d_out, sizeof(int) * tot);
   if (!rc) rc = fmbh_h2d(*d_out, h, sizeof(int) * tot, stream);
@@ -225,7 +228,8 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
     if (pl->prm.rcr_path == FMB_RCR_VALU) continue;
     if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
       break;
-    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], hoff, coff,
+    tb->nw[op] = 4;
+    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op], hoff, coff,
                           &tb->d_sched[op], &tb->sched_len[op], stream)))
       break;
     if (!d_coff) {
@@ -342,27 +346,29 @@ static int size_gws(fmb_plan *pl, int op, size_t min_per_wg) {
  * box pairs and buffers at nrhs_max right-hand sides, in every valid shape:
  * LDS mode with 8 or 16 vectors per batch (when the batch fits the 160 KB of
  * LDS), and global-workspace mode with 8 or 16 vectors and row groups of
- * 2, 3, 4 or 6 tiles.  The fastest shape (one warm-up, then one timed run,
- * HIP events on the plan's stream) replaces the fixed rule of
- * fmbh_rcr_shape. */
+ * 2, 3, 4 or 6 tiles, each with 4 or 8 waves per workgroup.  The fastest
+ * shape (one warm-up, then one timed run, HIP events on the plan's stream)
+ * replaces the fixed rule of fmbh_rcr_shape. */
 typedef struct {
-  int mode, nt, mt;
+  int mode, nt, mt, nw;
 } shape_t;
 
 static int shape_candidates(int P, int prec, shape_t *c) {
   int n = 0, mt = fmbh_rcr_mt_lds(P);
   static const int mts[4] = {2, 3, 4, 6};
-  for (int nt = 1; nt <= 2; nt++)
-    if (mt) {
-      shape_t v = {FMBH_RCR_LDS, nt, mt}, w = {FMBH_RCR_LDSP, nt, mt};
-      c[n++] = v;
-      c[n++] = w;
-    }
-  for (int nt = 1; nt <= 2; nt++)
-    for (int i = 0; i < 4; i++) {
-      shape_t v = {FMBH_RCR_GWS, nt, mts[i]};
-      c[n++] = v;
-    }
+  for (int nw = 4; nw <= 8; nw += 4) {
+    for (int nt = 1; nt <= 2; nt++)
+      if (mt) {
+        shape_t v = {FMBH_RCR_LDS, nt, mt, nw}, w = {FMBH_RCR_LDSP, nt, mt, nw};
+        c[n++] = v;
+        c[n++] = w;
+      }
+    for (int nt = 1; nt <= 2; nt++)
+      for (int i = 0; i < 4; i++) {
+        shape_t v = {FMBH_RCR_GWS, nt, mts[i], nw};
+        c[n++] = v;
+      }
+  }
   return n;
 }
 
@@ -436,8 +442,8 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
       int *coff = (int *)malloc(sizeof(int) * (mn + 1));
       coax_offsets(tb->pin, tb->pout, coff);
       tuned_t key = {o->prec, kind, tb->pin, tb->pout, pairs->nchunks, pl->nrhs_max,
-                     {0, 0, 0}, 0.0f}, hit;
-      shape_t c[16];
+                     {0, 0, 0, 0}, 0.0f}, hit;
+      shape_t c[32];
       int nc;
       float best = 0.0f;
       if (cache_find(&key, &hit)) {
@@ -450,11 +456,12 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         nc = shape_candidates(tb->P, o->prec, c);
       }
       for (int i = 0; i < nc && !rc; i++) {
-        if (c[i].mode == tb->mode[op] && c[i].nt == tb->nt[op] && c[i].mt == tb->mt[op])
+        if (c[i].mode == tb->mode[op] && c[i].nt == tb->nt[op] && c[i].mt == tb->mt[op] &&
+            c[i].nw == tb->nw[op])
           continue;
         int *sched = NULL, slen = 0;
-        if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, hoff, coff,
-                              &sched, &slen, stream)))
+        if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, c[i].nw, hoff,
+                              coff, &sched, &slen, stream)))
           break;
         if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP) {
           size_t b = fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
@@ -466,11 +473,12 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
           fmbh_free(sched);
           break;
         }
-        shape_t prev = {tb->mode[op], tb->nt[op], tb->mt[op]};
+        shape_t prev = {tb->mode[op], tb->nt[op], tb->mt[op], tb->nw[op]};
         int *psched = tb->d_sched[op], plen = tb->sched_len[op];
         tb->mode[op] = c[i].mode;
         tb->nt[op] = c[i].nt;
         tb->mt[op] = c[i].mt;
+        tb->nw[op] = c[i].nw;
         tb->d_sched[op] = sched;
         tb->sched_len[op] = slen;
         float ms;
@@ -482,6 +490,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
           tb->mode[op] = prev.mode;
           tb->nt[op] = prev.nt;
           tb->mt[op] = prev.mt;
+          tb->nw[op] = prev.nw;
           tb->d_sched[op] = psched;
           tb->sched_len[op] = plen;
           fmbh_free(sched);
@@ -493,6 +502,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         key.shape.mode = tb->mode[op];
         key.shape.nt = tb->nt[op];
         key.shape.mt = tb->mt[op];
+        key.shape.nw = tb->nw[op];
         key.ms = best;
         if (!cache_find(&key, &hit)) cache_put(&key);
       }
@@ -581,6 +591,7 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.mode = tb->mode[op];
   j.nt = tb->nt[op];
   j.mt = tb->mt[op];
+  j.nw = tb->nw[op];
   j.chunks = pairs->d_chunks;
   j.nchunks = pairs->nchunks;
   j.chunk_pairs = RCR_CHUNK;
@@ -644,6 +655,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.mode = tb->mode[op];
   j.nt = tb->nt[op];
   j.mt = tb->mt[op];
+  j.nw = tb->nw[op];
   j.chunks = pairs->d_chunks;
   j.nchunks = pairs->nchunks;
   j.chunk_pairs = RCR_CHUNK;
@@ -683,7 +695,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
 }
 
 int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
-                        int *nt, int *mt, double *ms) {
+                        int *nt, int *mt, int *nw, double *ms) {
   if (!pl || kind < 0 || kind > 2 || level < 2 || level > pl->tree->lmax ||
       (kind && level < 3))
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_tile_shape: invalid arguments");
@@ -692,6 +704,7 @@ int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *
   if (mode) *mode = tb->mode[op];
   if (nt) *nt = tb->nt[op];
   if (mt) *mt = tb->mt[op];
+  if (nw) *nw = tb->nw[op];
   if (ms) *ms = tb->tuned_ms[op];
   return FMB_OK;
 }

@@ -189,10 +189,11 @@ int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream);
  * level -> level-1, 2 L2L level-1 -> level) in the pass of precision prec:
  * mode (1: the batch in LDS, 2: in a global workspace, 3: in LDS on a
  * persistent grid, 0: VALU kernel), 16-column
- * tiles per batch nt (8 nt vectors), row tiles per unit mt, and with
- * autotune the timed milliseconds of one launch (else 0). */
+ * tiles per batch nt (8 nt vectors), row tiles per unit mt, waves per
+ * workgroup nw (4 or 8), and with autotune the timed milliseconds of one
+ * launch (else 0).  Any output pointer may be NULL. */
 int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
-                        int *nt, int *mt, double *ms);
+                        int *nt, int *mt, int *nw, double *ms);
 /* Wavenumber of the operator the plan holds (0 after a failed retune). */
 int fmb_plan_wavenumber(const fmb_plan *pl, double *k);
 /* Device bytes held by the plan. */

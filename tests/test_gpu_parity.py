@@ -242,13 +242,14 @@ def test_autotuned_shapes_give_the_same_operator():
         assert np.linalg.norm(a - b) / np.linalg.norm(a) < tol
         for kind in range(3):
             for l in range(2 if kind == 0 else 3, 6):
-                mode, nt, mt, ms = tuned.tile_shape(kind, l, prec)
+                mode, nt, mt, nw, ms = tuned.tile_shape(kind, l, prec)
                 assert mode in (1, 2, 3) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
-                assert ref.tile_shape(kind, l, prec)[3] == 0.0
+                assert nw in (4, 8)
+                assert ref.tile_shape(kind, l, prec)[3:] == (4, 0.0)
     again = F.Plan(t, k, autotune=1, **kw)
     for kind in range(3):
         for l in range(2 if kind == 0 else 3, 6):
-            assert again.tile_shape(kind, l, F.FMB_MIXED)[:3] == tuned.tile_shape(kind, l, F.FMB_MIXED)[:3]
+            assert again.tile_shape(kind, l, F.FMB_MIXED)[:4] == tuned.tile_shape(kind, l, F.FMB_MIXED)[:4]
 
 
 def test_config1_depth3_order8_vs_oracle_and_mie(sphere_l3):
