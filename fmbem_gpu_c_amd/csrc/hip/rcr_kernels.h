@@ -119,11 +119,9 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   const int ld = pad16(s), nks = (s + 3) >> 2, ng = (nks + KU - 1) / KU;
   const T *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  // set by the first K-step (every block has one), which multiplies into a
+  // zero accumulator operand instead of cleared registers
   acc_t acc[NTL][NT];
-#pragma unroll
-  for (int mt = 0; mt < NTL; mt++)
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
   T a0[KU][NTL], b0[KU][NT], a1[KU][NTL], b1[KU][NT];
   auto fetch = [&](int g, T(&fa)[KU][NTL], T(&fb)[KU][NT]) {
 #pragma unroll
@@ -141,7 +139,13 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
   auto mma = [&](int g, const T(&fa)[KU][NTL], const T(&fb)[KU][NT]) {
 #pragma unroll
     for (int u = 0; u < KU; u++) {
-      if (g * KU + u < nks) {
+      if (u == 0 && g == 0) {
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+          for (int t = 0; t < NT; t++)
+            acc[mt][t] = Mfma<T>::mma(fa[u][mt], fb[u][t], acc_t{0, 0, 0, 0});
+      } else if (g * KU + u < nks) {
 #pragma unroll
         for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
@@ -158,14 +162,24 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
       mma(g + 1, a1, b1);
     }
   }
+  // rows of a full tile are stored unguarded (a wave-uniform test)
 #pragma unroll
   for (int mt = 0; mt < NTL; mt++) {
+    const int r0 = 16 * (t0 + mt);
+    T *xo = Xo + (base + r0) * RS + li;
+    if (r0 + 16 <= s) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
-      if (i < s) {
+      for (int e = 0; e < 4; e++)
 #pragma unroll
-        for (int t = 0; t < NT; t++) Xo[(base + i) * RS + 16 * t + li] = acc[mt][t][e];
+        for (int t = 0; t < NT; t++) xo[Mfma<T>::row(lk, e) * RS + 16 * t] = acc[mt][t][e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int i = Mfma<T>::row(lk, e);
+        if (r0 + i < s) {
+#pragma unroll
+          for (int t = 0; t < NT; t++) xo[i * RS + 16 * t] = acc[mt][t][e];
+        }
       }
     }
   }
@@ -187,11 +201,8 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
   const int so = pout - am, si = pin - am;
   const int ld = pad16(so), nks = (si + 3) >> 2, ng = (nks + KU - 1) / KU;
   const cplx<T> *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  // set by the first K-step (every block has one), as in rot_unit
   acc_t acc[NTL][NB];
-#pragma unroll
-  for (int mt = 0; mt < NTL; mt++)
-#pragma unroll
-    for (int t = 0; t < NB; t++) acc[mt][t] = acc_t{0, 0, 0, 0};
   cplx<T> a0[KU][NTL], z0[KU][NB], a1[KU][NTL], z1[KU][NB];
   auto fetch = [&](int g, cplx<T>(&fa)[KU][NTL], cplx<T>(&fz)[KU][NB]) {
 #pragma unroll
@@ -219,13 +230,23 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
           b[t] = odd ? fz[u][t].im : fz[u][t].re;
           bq[t] = odd ? fz[u][t].re : -fz[u][t].im;
         }
+        if (u == 0 && g == 0) {
 #pragma unroll
-        for (int mt = 0; mt < NTL; mt++)
+          for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-          for (int t = 0; t < NB; t++) {
-            acc[mt][t] = Mfma<T>::mma(fa[u][mt].re, b[t], acc[mt][t]);
-            acc[mt][t] = Mfma<T>::mma(fa[u][mt].im, bq[t], acc[mt][t]);
-          }
+            for (int t = 0; t < NB; t++) {
+              acc[mt][t] = Mfma<T>::mma(fa[u][mt].re, b[t], acc_t{0, 0, 0, 0});
+              acc[mt][t] = Mfma<T>::mma(fa[u][mt].im, bq[t], acc[mt][t]);
+            }
+        } else {
+#pragma unroll
+          for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+            for (int t = 0; t < NB; t++) {
+              acc[mt][t] = Mfma<T>::mma(fa[u][mt].re, b[t], acc[mt][t]);
+              acc[mt][t] = Mfma<T>::mma(fa[u][mt].im, bq[t], acc[mt][t]);
+            }
+        }
       }
     }
   };
@@ -238,18 +259,26 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
       mma(g + 1, a1, z1);
     }
   }
+  // rows of a full tile are stored unguarded (a wave-uniform test)
+  auto put = [&](int mt, int e, int i) {
+    const int nn = am + i, rb = nn * nn + nn;
+#pragma unroll
+    for (int sg = 0; sg < SG; sg++)
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        Xo[(rb + (sg ? -am : am)) * RS + 16 * t + li] = acc[mt][sg * NT + t][e];
+  };
 #pragma unroll
   for (int mt = 0; mt < NTL; mt++) {
+    const int r0 = 16 * (t0 + mt);
+    if (r0 + 16 <= so) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
-      if (i < so) {
-        const int nn = am + i, rb = nn * nn + nn;
+      for (int e = 0; e < 4; e++) put(mt, e, r0 + Mfma<T>::row(lk, e));
+    } else {
 #pragma unroll
-        for (int sg = 0; sg < SG; sg++)
-#pragma unroll
-          for (int t = 0; t < NT; t++)
-            Xo[(rb + (sg ? -am : am)) * RS + 16 * t + li] = acc[mt][sg * NT + t][e];
+      for (int e = 0; e < 4; e++) {
+        const int i = r0 + Mfma<T>::row(lk, e);
+        if (i < so) put(mt, e, i);
       }
     }
   }
