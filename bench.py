@@ -63,16 +63,23 @@ def timed_matvecs(torch, plan, x, y, prec, steps, warmup, flops):
     torch.cuda.synchronize()
     plan.set_timing(True)
     m2l_ms = {l: 0.0 for l in flops}
+    # matvec phases (HIP events on the plan's stream): gather + P2M, M2M,
+    # M2L + L2L, L2P + near field + scatter
+    names = {1: "gather_p2m", 2: "m2m", 3: "m2l_l2l", 5: "l2p_near"}
+    phase_ms = {v: 0.0 for v in names.values()}
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.matvec(x, y, prec=prec)
-        _, m2l = plan.timing()
+        ph, m2l = plan.timing()
         for l in flops:
             m2l_ms[l] += m2l[l]
+        for i, nm in names.items():
+            phase_ms[nm] += ph[i]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     plan.set_timing(False)
-    return dt, {l: v / steps for l, v in m2l_ms.items()}
+    return (dt, {l: v / steps for l, v in m2l_ms.items()},
+            {nm: v / steps for nm, v in phase_ms.items()})
 
 
 def cpu_baseline(V, F_, k, depth, stride):
@@ -158,8 +165,8 @@ def main():
     y = torch.empty_like(x)
     flops = m2l_flops(tree, plan, nrhs)
 
-    dt32, m2l32 = timed_matvecs(torch, plan, x, y, F.FMB_MIXED, a.steps, a.warmup, flops)
-    dt64, m2l64 = timed_matvecs(torch, plan, x, y, F.FMB_FP64, a.steps, a.warmup, flops)
+    dt32, m2l32, ph32 = timed_matvecs(torch, plan, x, y, F.FMB_MIXED, a.steps, a.warmup, flops)
+    dt64, m2l64, ph64 = timed_matvecs(torch, plan, x, y, F.FMB_FP64, a.steps, a.warmup, flops)
     mv32 = a.steps * nrhs / dt32
     mv64 = a.steps * nrhs / dt64
     log("mixed %.3f ms/step, fp64 %.3f ms/step" % (1e3 * dt32 / a.steps, 1e3 * dt64 / a.steps))
@@ -243,6 +250,7 @@ def main():
                      "fp64_achieved": ach64, "fp64_frac": ach64 / PEAK["fp64"],
                      "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
                      "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},
+        "phases_ms": {"fp32": ph32, "fp64": ph64},
         "cpu_baseline": cpu,
         "tile_shapes": {
             name: {str(l): list(plan.tile_shape(0, l, prec)[:4]) for l in range(2, w["depth"] + 1)}
