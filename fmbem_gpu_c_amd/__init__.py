@@ -50,6 +50,7 @@ class PlanParams(ctypes.Structure):
         ("k_max", ctypes.c_double),
         ("autotune", ctypes.c_int),
         ("serial_levels", ctypes.c_int),
+        ("waves", ctypes.c_int),
     ]
 
 

@@ -485,6 +485,8 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: quadrature parameters");
   if (prm->rcr_path != FMB_RCR_AUTO && prm->rcr_path != FMB_RCR_VALU)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
+  if (prm->waves != 0 && prm->waves != 4 && prm->waves != 8)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: waves %d (0, 4 or 8)", prm->waves);
   int rc;
   fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
   pl->tree = t;

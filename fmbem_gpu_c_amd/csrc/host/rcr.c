@@ -228,7 +228,7 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
     if (pl->prm.rcr_path == FMB_RCR_VALU) continue;
     if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
       break;
-    tb->nw[op] = 4;
+    tb->nw[op] = pl->prm.waves ? pl->prm.waves : 4;
     if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op], hoff, coff,
                           &tb->d_sched[op], &tb->sched_len[op], stream)))
       break;

@@ -160,7 +160,11 @@ typedef struct fmb_plan_params {
   int serial_levels;         /* 1: M2L levels one after another on the
                                 caller's stream; 0: the coarse levels and
                                 the L2L chain on a second stream, beside
-                                the three finest M2L levels                 */
+                                the M2L of levels L-2 and L-1 (the finest
+                                M2L then runs alone)                        */
+  int waves;                 /* waves per workgroup of the matrix-core
+                                translations in the fixed tile-shape rule:
+                                4 or 8 (0: 4); autotune overrides it        */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);

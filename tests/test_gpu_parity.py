@@ -284,9 +284,10 @@ def test_config1_depth3_order8_vs_oracle_and_mie(sphere_l3):
 @pytest.mark.parametrize("freq", [4000.0, 12000.0])
 def test_concurrent_levels_match_serial(freq):
     """The coarse M2L levels and the L2L chain on a second stream beside
-    the three finest M2L levels (the default) against all levels in order on
-    one stream: the 20,480-triangle sphere at depth 6, FP64 pass 1e-13, FP32
-    pass 1e-6 relative (summation order only), twice in a row."""
+    the M2L of levels L-2 and L-1, then the finest M2L alone (the default),
+    against all levels in order on one stream: the 20,480-triangle sphere at
+    depth 6, FP64 pass 1e-13, FP32 pass 1e-6 relative (summation order
+    only), twice in a row."""
     m = F.Mesh.icosphere(5, 0.1)
     t = F.Tree(m, depth=6)
     k = wavenumber(freq)
@@ -301,3 +302,43 @@ def test_concurrent_levels_match_serial(freq):
         for _ in range(2):
             b = con.matvec(x, prec=prec).cpu().numpy()
             assert np.linalg.norm(a - b) / np.linalg.norm(a) < tol
+
+
+@pytest.mark.parametrize("freq", [4000.0, 12000.0])
+def test_eight_wave_shapes_match_four_wave(freq):
+    """Every matrix-core translation with 8 waves per workgroup (the fixed
+    tile rule otherwise: LDS batches at low orders, global-workspace batches
+    at high ones) against 4 waves: the 20,480-triangle sphere at depth 6,
+    forward and transposed passes, FP64 1e-13 and FP32 1e-6 relative (the
+    unit schedule changes only the order of the target sums)."""
+    m = F.Mesh.icosphere(5, 0.1)
+    t = F.Tree(m, depth=6)
+    k = wavenumber(freq)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=2, serial_levels=1)
+    p4 = F.Plan(t, k, **kw)
+    p8 = F.Plan(t, k, waves=8, **kw)
+    _, L, _, _ = t.info()
+    for kind in range(3):
+        for l in range(2 if kind == 0 else 3, L + 1):
+            for prec in (F.FMB_FP64, F.FMB_MIXED):
+                assert p4.tile_shape(kind, l, prec)[3] == 4
+                assert p8.tile_shape(kind, l, prec)[3] == 8
+    modes = {p8.tile_shape(0, l, F.FMB_FP64)[0] for l in range(2, L + 1)}
+    print("f", freq, "orders", p8.orders()[2:L + 1].tolist(), "M2L modes", sorted(modes), flush=True)
+    n = m.arrays()[1].shape[0]
+    rng = np.random.default_rng(43)
+    x = torch.tensor(rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)), device="cuda")
+    for prec, tol in ((F.FMB_FP64, 1e-13), (F.FMB_MIXED, 1e-6)):
+        a = p4.matvec(x, prec=prec).cpu().numpy()
+        b = p8.matvec(x, prec=prec).cpu().numpy()
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < tol, (prec, np.linalg.norm(a - b) / np.linalg.norm(a))
+    a = p4.matvec_t(x).cpu().numpy()
+    b = p8.matvec_t(x).cpu().numpy()
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-13
+
+
+def test_waves_parameter_is_checked():
+    m = F.Mesh.icosphere(2, 0.1)
+    t = F.Tree(m, depth=2)
+    with pytest.raises(F.FmbError):
+        F.Plan(t, wavenumber(1000.0), waves=6)
