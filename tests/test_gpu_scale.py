@@ -1,0 +1,79 @@
+"""The full-size workload: the 327,680-triangle icosphere (r = 0.5 m) at
+10 kHz, depth 7, orders 94 62 42 31 23 18, checked through properties that
+hold at any size (the oracle's dense or FMM matvec at this size would take
+minutes of CPU time):
+
+* the FP32 pass against the FP64 pass, 1e-6 relative;
+* the transpose identity y . (A x) = (A^T y) . x, 1e-12 of |y| |A x|;
+* two FP64 matvecs in a row, 1e-13 (target atomics change the order of the
+  sums only);
+* GMRES to 1e-6 in both work precisions: the solutions agree to 1e-5, and
+  the FP64 surface pressure meets the rigid-sphere Mie series to 1%
+  (0.6% is the discretisation error of flat panels at this mesh)."""
+import numpy as np
+import pytest
+import torch
+
+import fmbem_gpu_c_amd as F
+from conftest import wavenumber
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sphere7():
+    m = F.Mesh.icosphere(7, 0.5)
+    t = F.Tree(m, depth=7)
+    k = wavenumber(10000.0)
+    pl = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=2)
+    return m, t, k, pl
+
+
+def _rand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def test_full_size_matvec_properties(sphere7):
+    m, t, k, pl = sphere7
+    assert pl.orders()[2:8].tolist() == [94, 62, 42, 31, 23, 18]
+    n = m.arrays()[1].shape[0]
+    assert n == 327680
+    x, y = _rand((2, n), 71), _rand((2, n), 72)
+    xt, yt = torch.tensor(x, device="cuda"), torch.tensor(y, device="cuda")
+    a64 = pl.matvec(xt, prec=F.FMB_FP64).cpu().numpy()
+    a32 = pl.matvec(xt, prec=F.FMB_MIXED).cpu().numpy()
+    err = np.linalg.norm(a32 - a64) / np.linalg.norm(a64)
+    print("FP32 pass vs FP64 pass %.2e" % err, flush=True)
+    assert err < 1e-6
+    again = pl.matvec(xt, prec=F.FMB_FP64).cpu().numpy()
+    assert np.linalg.norm(again - a64) / np.linalg.norm(a64) < 1e-13
+    aty = pl.matvec_t(yt, prec=F.FMB_FP64).cpu().numpy()
+    for r in range(2):
+        lhs, rhs = np.sum(y[r] * a64[r]), np.sum(aty[r] * x[r])
+        scale = np.linalg.norm(y[r]) * np.linalg.norm(a64[r])
+        print("rhs %d transpose identity %.2e" % (r, abs(lhs - rhs) / scale), flush=True)
+        assert abs(lhs - rhs) < 1e-12 * scale
+
+
+def test_full_size_solve_vs_mie(sphere7):
+    import oracle as O
+    m, t, k, pl = sphere7
+    V, Fc = m.arrays()
+    b = pl.rhs_plane_wave(np.array([[0.0, 0.0, 1.0]]))
+    u = {}
+    for name, prec in (("fp64", F.FMB_FP64), ("mixed", F.FMB_MIXED)):
+        pl.set_work_precision(prec)
+        x, info = pl.gmres(b, restart=30, tol=1e-6, maxit=300)
+        print(name, "iters", info.iters, "relres %.2e" % info.relres[0], flush=True)
+        assert info.relres[0] <= 1e-6
+        u[name] = x.cpu().numpy()[0]
+    pl.set_work_precision(F.FMB_FP64)
+    err = np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"])
+    c = V[Fc].mean(1)
+    ct = c[:, 2] / np.linalg.norm(c, axis=1)
+    mie = O.mie_rigid(k * 0.5, ct, nterms=int(k * 0.5) + 40)
+    emie = np.linalg.norm(u["fp64"] - mie) / np.linalg.norm(mie)
+    print("mixed vs FP64 %.2e, FP64 vs Mie %.4f" % (err, emie), flush=True)
+    assert err < 1e-5
+    assert emie < 0.01
