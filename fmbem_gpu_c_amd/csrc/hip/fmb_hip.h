@@ -79,7 +79,7 @@ int fmbh_scatter(const fmb_plan *pl, int op, fmb_c64 *y, int nrhs,
 int fmbh_p2m(const fmb_plan *pl, int op, int nrhs, void *stream);
 int fmbh_m2m(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
-/* matrix-core RCR translations (rcr_mfma.hip) */
+/* matrix-core RCR translations (rcr_kernels.h, rcr_mfma.hip) */
 #define FMBH_RCR_LDS 1 /* batch in LDS, blocks owned by one wave          */
 #define FMBH_RCR_GWS 2 /* batch in a global workspace, row-group units    */
 #define FMBH_RCR_LDSP 3 /* as LDS, a persistent grid looping over batches */

@@ -1,5 +1,5 @@
 /* rcr.c -- host planning and dispatch of the rotate-coaxial-rotate
- * translations on the matrix cores (csrc/hip/rcr_mfma.hip).
+ * translations on the matrix cores (csrc/hip/rcr_kernels.h, rcr_mfma.hip).
  *
  * Every translation kind groups its box pairs into buckets that share one
  * operator:

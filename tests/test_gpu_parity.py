@@ -161,7 +161,7 @@ def _far_field(pl, x, prec):
                                          (4000.0, [0, 0, 47, 40, 31, 25]),
                                          (16000.0, [0, 0, 62, 44, 30, 20])])
 def test_rcr_matrix_cores_match_valu(freq, orders):
-    """The matrix-core translations (rcr_mfma.hip: M2L, M2M, L2L) against the
+    """The matrix-core translations (rcr_kernels.h: M2L, M2M, L2L) against the
     one-translation-per-workgroup VALU kernels on the same plan: far field
     only (P2M .. L2P), three right-hand sides so batches straddle box pairs,
     level orders rising and falling so M2M / L2L run with pin != pout, and
