@@ -347,8 +347,8 @@ static int size_gws(fmb_plan *pl, int op, size_t min_per_wg) {
  * LDS mode with 8 or 16 vectors per batch (when the batch fits the 160 KB of
  * LDS), and global-workspace mode with 8 or 16 vectors and row groups of
  * 2, 3, 4 or 6 tiles, each with 4 or 8 waves per workgroup.  The fastest
- * shape (one warm-up, then one timed run, HIP events on the plan's stream)
- * replaces the fixed rule of fmbh_rcr_shape. */
+ * shape (a one-vector warm-up, then one timed run, HIP events on the plan's
+ * stream) replaces the fixed rule of fmbh_rcr_shape. */
 typedef struct {
   int mode, nt, mt, nw;
 } shape_t;
@@ -372,11 +372,13 @@ static int shape_candidates(int P, int prec, shape_t *c) {
   return n;
 }
 
+/* one launch at one right-hand side (loads the kernel's code object and the
+ * operator blocks into the caches), then one timed launch at nrhs_max */
 static int time_run(fmb_plan *pl, int op, int kind, int l, void *stream, float *ms) {
   void *e0 = NULL, *e1 = NULL;
   int rc = fmbh_event_create(&e0);
   if (!rc) rc = fmbh_event_create(&e1);
-  if (!rc) rc = fmbx_rcr_run(pl, op, kind, l, pl->nrhs_max, stream);
+  if (!rc) rc = fmbx_rcr_run(pl, op, kind, l, 1, stream);
   if (!rc) rc = fmbh_event_record(e0, stream);
   if (!rc) rc = fmbx_rcr_run(pl, op, kind, l, pl->nrhs_max, stream);
   if (!rc) rc = fmbh_event_record(e1, stream);
