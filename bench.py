@@ -56,6 +56,19 @@ def m2l_flops(tree, plan, nrhs):
     return out
 
 
+def m2m_l2l_flops(tree, plan, nrhs):
+    """RCR algorithmic flops of the upward and downward passes per matvec:
+    sum over l of 2 B_l * 8 (10 p_{l-1}^3 - p_{l-1}) / 3 per RHS (one M2M and
+    one L2L per box, counted at the parent's order)."""
+    p = plan.orders()
+    _, L, _, _ = tree.info()
+    tot = 0.0
+    for l in range(3, L + 1):
+        B = len(tree.level(l))
+        tot += 2 * B * 8.0 * (10.0 * p[l - 1] ** 3 - p[l - 1]) / 3.0 * nrhs
+    return tot
+
+
 def timed_matvecs(torch, plan, x, y, prec, steps, warmup, flops):
     import fmbem_gpu_c_amd as F
     for _ in range(warmup):
@@ -164,6 +177,7 @@ def main():
     x = torch.randn(nrhs, n, dtype=torch.complex128, device="cuda")
     y = torch.empty_like(x)
     flops = m2l_flops(tree, plan, nrhs)
+    flops_mv = sum(flops.values()) + m2m_l2l_flops(tree, plan, nrhs)
 
     dt32, m2l32, ph32 = timed_matvecs(torch, plan, x, y, F.FMB_MIXED, a.steps, a.warmup, flops)
     dt64, m2l64, ph64 = timed_matvecs(torch, plan, x, y, F.FMB_FP64, a.steps, a.warmup, flops)
@@ -251,6 +265,10 @@ def main():
                      "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
                      "m2l_ms_per_level_fp64": {str(l): v for l, v in m2l64.items()}},
         "phases_ms": {"fp32": ph32, "fp64": ph64},
+        # whole matvec: counted RCR flops (M2L, M2M, L2L) over the step time
+        "matvec_rcr_flops": flops_mv,
+        "matvec_achieved_tflops": {"fp32": flops_mv / dt32 * a.steps / 1e12,
+                                   "fp64": flops_mv / dt64 * a.steps / 1e12},
         "cpu_baseline": cpu,
         "tile_shapes": {
             name: {str(l): list(plan.tile_shape(0, l, prec)[:4]) for l in range(2, w["depth"] + 1)}
