@@ -194,6 +194,7 @@ def main():
     lbig = max(m2l32, key=lambda l: m2l32[l])
     ach32 = flops[lbig] / (m2l32[lbig] * 1e-3) / 1e12
     ach64 = flops[lbig] / (m2l64[lbig] * 1e-3) / 1e12
+    traffic = traffic_of(a.workload, lbig)
 
     solve = {}
     if not a.no_solve:
@@ -259,7 +260,9 @@ def main():
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": ach32 / PEAK["fp32"],
-                     "traffic": traffic_of(a.workload, lbig),
+                     "traffic": traffic,
+                     # HBM rate of that launch: its counted bytes over its time
+                     "hbm_GBps": (traffic / (m2l32[lbig] * 1e-3) / 1e9) if traffic else None,
                      "kernel": "M2L level %d, FP32 matrix-core kernel" % lbig,
                      "fp64_achieved": ach64, "fp64_frac": ach64 / PEAK["fp64"],
                      "m2l_ms_per_level_fp32": {str(l): v for l, v in m2l32.items()},
