@@ -133,7 +133,8 @@ extern "C" int fmbh_capture_end(void *stream, void **exec) {
   FMB_HIP_CHECK(hipStreamEndCapture((hipStream_t)stream, &g));
   hipGraphExec_t e = NULL;
   hipError_t err = hipGraphInstantiate(&e, g, NULL, NULL, 0);
-  hipGraphDestroy(g);
+  /* the graph is released whatever its instantiation gave */
+  (void)hipGraphDestroy(g);
   if (err != hipSuccess)
     return fmb_set_error(FMB_ERR_HIP, "graph instantiation: %s", hipGetErrorString(err));
   *exec = e;

@@ -63,6 +63,10 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
   /* p at the field points, J, u = conj(p) */
   if ((rc = fmbh_field(pl, x, nrhs, d_pts, npts, d_dirs, d_p, stream))) goto done;
   h_p = (fmb_c64 *)malloc(sizeof(fmb_c64) * np);
+  if (!h_p) {
+    rc = fmb_set_error(FMB_ERR_NOMEM, "fmb_adjoint_grad: host buffer");
+    goto done;
+  }
   if ((rc = fmbh_d2h(h_p, d_p, sizeof(fmb_c64) * np, stream))) goto done;
   {
     double s = 0.0;

@@ -131,6 +131,7 @@ static int sys_init(sys_t *S, fmb_plan *const *pls, const int *nrs, int np, int 
   S->N = pls[0]->tree->nelem;
   S->off = (int *)malloc(sizeof(int) * (np + 1));
   S->live = (int *)malloc(sizeof(int) * np);
+  if (!S->off || !S->live) return fmb_set_error(FMB_ERR_NOMEM, "GMRES system");
   S->off[0] = 0;
   for (int p = 0; p < np; p++) {
     S->off[p + 1] = S->off[p] + nrs[p];
@@ -146,6 +147,7 @@ static int sys_graph_init(sys_t *S, fmb_c64 *ys) {
   S->ys = ys;
   S->bs = (void **)calloc(S->np, sizeof(void *));
   S->ev_join = (void **)calloc(S->np, sizeof(void *));
+  if (!S->bs || !S->ev_join) return fmb_set_error(FMB_ERR_NOMEM, "GMRES graph branches");
   if ((rc = fmbh_stream_create(&S->cs))) return rc;
   if ((rc = fmbh_event_create(&S->ev_fork))) return rc;
   for (int p = 0; p < S->np; p++) {
@@ -239,6 +241,11 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
   rsave = (double *)malloc(sizeof(double) * nrhs);
   active = (int *)malloc(sizeof(int) * nrhs);
   sel = (int *)malloc(sizeof(int) * nrhs);
+  if (!Hh || !cs || !sn || !g || !yv || !hh || !hh2 || !bn || !rn || !hn || !rsave || !active ||
+      !sel) {
+    rc = fmb_set_error(FMB_ERR_NOMEM, "GMRES host workspace");
+    goto done;
+  }
 #define HH(r, i, j) Hh[((size_t)(r) * m + (j)) * (m + 1) + (i)]
   if ((rc = fmbh_znorms(x, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;

@@ -199,11 +199,10 @@ typedef struct {
   int nv, cap;
 } vbuf;
 
+/* the callers size the buffer exactly (12 icosahedron vertices, then
+ * 10 4^level + 2 for a subdivided icosphere) */
 static int vpush(vbuf *b, const double *p) {
-  if (b->nv == b->cap) {
-    b->cap = b->cap ? 2 * b->cap : 64;
-    b->V = (double *)realloc(b->V, sizeof(double) * 3 * b->cap);
-  }
+  if (b->nv == b->cap) return -1;
   memcpy(b->V + 3 * b->nv, p, sizeof(double) * 3);
   return b->nv++;
 }
@@ -290,9 +289,17 @@ static void orient_faces_radially(const double *V, int *F, int nf) {
 int fmb_mesh_icosphere(int level, double radius, fmb_mesh **out) {
   if (level < 0 || level > 10 || !(radius > 0) || !out)
     return fmb_set_error(FMB_ERR_ARG, "icosphere level %d radius %g", level, radius);
+  /* 10 4^level + 2 vertices in all: the buffer never grows */
   vbuf vb = {0};
+  vb.cap = 10 * (1 << (2 * level)) + 2;
+  vb.V = (double *)malloc(sizeof(double) * 3 * (size_t)vb.cap);
   int nf = 20;
   int *F = (int *)malloc(sizeof(int) * 3 * 20);
+  if (!vb.V || !F) {
+    free(vb.V);
+    free(F);
+    return fmb_set_error(FMB_ERR_NOMEM, "icosphere level %d", level);
+  }
   icosahedron(&vb, F);
   for (int l = 0; l < level; l++) {
     emap em;
@@ -300,8 +307,16 @@ int fmb_mesh_icosphere(int level, double radius, fmb_mesh **out) {
     while (em.cap < (size_t)nf * 4) em.cap <<= 1;
     em.key = (long long *)malloc(sizeof(long long) * em.cap);
     em.val = (int *)malloc(sizeof(int) * em.cap);
+    int *G = (int *)malloc(sizeof(int) * 3 * 4 * (size_t)nf);
+    if (!em.key || !em.val || !G) {
+      free(em.key);
+      free(em.val);
+      free(G);
+      free(F);
+      free(vb.V);
+      return fmb_set_error(FMB_ERR_NOMEM, "icosphere level %d", level);
+    }
     for (size_t i = 0; i < em.cap; i++) em.key[i] = -1;
-    int *G = (int *)malloc(sizeof(int) * 3 * 4 * nf);
     for (int f = 0; f < nf; f++) {
       int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
       int ab = emap_get(&em, a, b, &vb), bc = emap_get(&em, b, c, &vb),
@@ -323,6 +338,11 @@ int fmb_mesh_icosphere(int level, double radius, fmb_mesh **out) {
   }
   orient_faces_radially(vb.V, F, nf);
   fmb_mesh *m = (fmb_mesh *)calloc(1, sizeof(fmb_mesh));
+  if (!m) {
+    free(F);
+    free(vb.V);
+    return fmb_set_error(FMB_ERR_NOMEM, "icosphere level %d", level);
+  }
   m->nv = vb.nv;
   m->nf = nf;
   m->V = vb.V;
@@ -379,6 +399,9 @@ int fmb_mesh_ellipsoid(int nu, double a, double b, double c, int lmin,
     return fmb_set_error(FMB_ERR_ARG, "ellipsoid parameters");
   /* geodesic sphere: each icosahedron face split into nu^2 triangles */
   vbuf ib = {0};
+  ib.cap = 12;
+  ib.V = (double *)malloc(sizeof(double) * 3 * 12);
+  if (!ib.V) return fmb_set_error(FMB_ERR_NOMEM, "ellipsoid mesh");
   int IF[60];
   icosahedron(&ib, IF);
   int nvmax = 10 * nu * nu + 2;
@@ -390,8 +413,17 @@ int fmb_mesh_ellipsoid(int nu, double a, double b, double c, int lmin,
   while (cap < (size_t)nvmax * 4) cap <<= 1;
   long long *hk = (long long *)malloc(sizeof(long long) * cap * 3);
   int *hv = (int *)malloc(sizeof(int) * cap);
-  for (size_t i = 0; i < cap; i++) hv[i] = -1;
   int *grid = (int *)malloc(sizeof(int) * (nu + 1) * (nu + 1));
+  if (!V || !F || !hk || !hv || !grid) {
+    free(V);
+    free(F);
+    free(hk);
+    free(hv);
+    free(grid);
+    free(ib.V);
+    return fmb_set_error(FMB_ERR_NOMEM, "ellipsoid mesh, %d subdivisions", nu);
+  }
+  for (size_t i = 0; i < cap; i++) hv[i] = -1;
   for (int f = 0; f < 20; f++) {
     const double *A = ib.V + 3 * IF[3 * f], *B = ib.V + 3 * IF[3 * f + 1],
                  *C = ib.V + 3 * IF[3 * f + 2];
@@ -458,6 +490,13 @@ int fmb_mesh_ellipsoid(int nu, double a, double b, double c, int lmin,
   for (int l = lmin; l <= lmax; l++)
     for (int m = -l; m <= l; m++) cf[l * l + l + m] = gauss01(&s);
   double *fv = (double *)malloc(sizeof(double) * nv);
+  if (!cf || !fv) {
+    free(cf);
+    free(fv);
+    free(V);
+    free(F);
+    return fmb_set_error(FMB_ERR_NOMEM, "ellipsoid perturbation");
+  }
   double ss = 0;
   for (int i = 0; i < nv; i++) {
     double *p = V + 3 * i;
@@ -477,6 +516,11 @@ int fmb_mesh_ellipsoid(int nu, double a, double b, double c, int lmin,
   free(cf);
   free(fv);
   fmb_mesh *m = (fmb_mesh *)calloc(1, sizeof(fmb_mesh));
+  if (!m) {
+    free(V);
+    free(F);
+    return fmb_set_error(FMB_ERR_NOMEM, "ellipsoid mesh");
+  }
   m->nv = nv;
   m->nf = fi;
   m->V = V;

@@ -14,6 +14,7 @@ int fmb_gen_rot_mats(const double *angles, int n, int p, fmb_c64 *out) {
   if (rc) return rc;
   size_t rs = (size_t)fmb_rot_size(p);
   double *betas = (double *)malloc(sizeof(double) * n);
+  if (!betas) return fmb_set_error(FMB_ERR_NOMEM, "fmb_gen_rot_mats");
   for (int i = 0; i < n; i++) betas[i] = angles[3 * i + 1];
   double *d_H = NULL;
   fmb_c64 *d_out = NULL;

@@ -258,6 +258,7 @@ static int plan_geometry(fmb_plan *pl, void *stream) {
     if (ch > 16) ch = 16;
     int cap = np / ch + FMB_NOFF + 1, n = 0;
     int *c = (int *)malloc(sizeof(int) * 4 * cap);
+    if (!c) return fmb_set_error(FMB_ERR_NOMEM, "M2L chunks");
     for (int o = 0; o < FMB_NOFF && np > 0; o++)
       for (int s = t->h_off_start[l][o]; s < t->h_off_start[l][o + 1]; s += ch) {
         int e = s + ch < t->h_off_start[l][o + 1] ? s + ch : t->h_off_start[l][o + 1];
@@ -300,6 +301,12 @@ static int plan_geometry(fmb_plan *pl, void *stream) {
   double *gx = (double *)malloc(sizeof(double) * ne),
          *gw = (double *)malloc(sizeof(double) * ne),
          *e = (double *)malloc(sizeof(double) * 2 * ne);
+  if (!gx || !gw || !e) {
+    free(gx);
+    free(gw);
+    free(e);
+    return fmb_set_error(FMB_ERR_NOMEM, "edge rule");
+  }
   fmbx_gauss_legendre(ne, gx, gw);
   for (int i = 0; i < ne; i++) {
     e[2 * i] = gx[i];
@@ -415,6 +422,10 @@ static int plan_freq(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream)
     double x = k * t->D * ldexp(1.0, -l) * sqrt(3.0) / 2.0;
     int n1 = pl->Pmax + 1;
     pl->h_scale[l] = (double *)malloc(sizeof(double) * n1);
+    if (!pl->h_scale[l]) {
+      rc = fmb_set_error(FMB_ERR_NOMEM, "scale factors");
+      goto out;
+    }
     fmbx_abs_hn(n1 - 1, x, pl->h_scale[l]);
     for (int n = 0; n < n1; n++) {
       double w = 1.0 / (x * pl->h_scale[l][n]);
@@ -432,7 +443,8 @@ static int plan_freq(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream)
     fmb_c64 *h = (fmb_c64 *)malloc(sizeof(fmb_c64) * N),
             *hs = (fmb_c64 *)malloc(sizeof(fmb_c64) * N);
     int *perm = (int *)malloc(sizeof(int) * N);
-    rc = fmbh_d2h(h, robin, sizeof(fmb_c64) * N, stream);
+    rc = (h && hs && perm) ? FMB_OK : fmb_set_error(FMB_ERR_NOMEM, "Robin coefficients");
+    if (!rc) rc = fmbh_d2h(h, robin, sizeof(fmb_c64) * N, stream);
     if (!rc) rc = fmbh_d2h(perm, t->d_perm, sizeof(int) * N, stream);
     if (!rc)
       for (int i = 0; i < N; i++) hs[i] = h[perm[i]];
@@ -489,6 +501,7 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: waves %d (0, 4 or 8)", prm->waves);
   int rc;
   fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
+  if (!pl) return fmb_set_error(FMB_ERR_NOMEM, "fmb_plan_create");
   pl->tree = t;
   pl->prm = *prm;
   pl->stream = stream;

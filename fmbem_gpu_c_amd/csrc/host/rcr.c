@@ -33,6 +33,7 @@ typedef struct {
 
 typedef struct {
   int code, cost, aoff; /* aoff: offset of the unit's padded operator block */
+  int w;                /* wave the schedule gives it */
 } unit_t;
 
 static int by_cost(const void *a, const void *b) {
@@ -47,14 +48,13 @@ static int by_cost(const void *a, const void *b) {
  * beginning at base) into off */
 static void lpt(unit_t *u, int n, int nw, int base, int *off, int *out) {
   int load[RCR_WAVES_MAX] = {0}, cnt[RCR_WAVES_MAX] = {0};
-  int *w_of = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   qsort(u, n, sizeof(unit_t), by_cost);
   for (int i = 0; i < n; i++) {
     int w = 0;
     for (int q = 1; q < nw; q++)
       if (load[q] < load[w]) w = q;
     load[w] += u[i].cost;
-    w_of[i] = w;
+    u[i].w = w;
     cnt[w]++;
   }
   off[0] = base;
@@ -62,11 +62,10 @@ static void lpt(unit_t *u, int n, int nw, int base, int *off, int *out) {
   int fill[RCR_WAVES_MAX];
   for (int w = 0; w < nw; w++) fill[w] = off[w];
   for (int i = 0; i < n; i++) {
-    int q = fill[w_of[i]]++;
+    int q = fill[u[i].w]++;
     out[2 * q] = u[i].code;
     out[2 * q + 1] = u[i].aoff;
   }
-  free(w_of);
 }
 
 static int tiles(int s) { return (s + 15) / 16; }
@@ -139,11 +138,16 @@ static int build_sched(int pin, int pout, int P, int mode, int mt, int nw, const
     return fmb_set_error(FMB_ERR_RANGE, "translation schedule for %d waves", nw);
   const int hdr = (3 * (nw + 1) + 1) & ~1;
   unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + 2 * P + pout));
+  if (!u) return fmb_set_error(FMB_ERR_NOMEM, "translation schedule");
   int n1 = rot_units(pin, split, mt, hoff, u);
   int nc = coax_units(pin, pout, split, mt, coff, u + n1);
   int n3 = rot_units(pout, split, mt, hoff, u + n1 + nc);
   int tot = hdr + 2 * (n1 + nc + n3);
   int *h = (int *)calloc(tot, sizeof(int));
+  if (!h) {
+    free(u);
+    return fmb_set_error(FMB_ERR_NOMEM, "translation schedule");
+  }
   int base = hdr / 2;
   lpt(u, n1, nw, base, h, h);
   base += n1;
@@ -163,6 +167,7 @@ static int build_chunks(const int *start, int nbucket, fmb_rcr_tab *tb,
                         void *stream) {
   int npairs = start[nbucket] - start[0], n = 0, rc;
   int *c = (int *)malloc(sizeof(int) * 4 * (npairs / RCR_CHUNK + nbucket + 1));
+  if (!c) return fmb_set_error(FMB_ERR_NOMEM, "translation chunks");
   for (int b = 0; b < nbucket; b++)
     for (int s = start[b]; s < start[b + 1]; s += RCR_CHUNK) {
       int e = s + RCR_CHUNK < start[b + 1] ? s + RCR_CHUNK : start[b + 1];
@@ -185,7 +190,8 @@ static int octant_pairs(const fmb_tree *t, int l, fmb_rcr_tab *tb, void *stream)
   int *key = (int *)malloc(sizeof(int) * nb), *par = (int *)malloc(sizeof(int) * nb);
   int *sp = (int *)malloc(sizeof(int) * nb), *sc = (int *)malloc(sizeof(int) * nb);
   int start[9] = {0};
-  rc = fmbh_d2h(key, t->d_keys[l], sizeof(int) * nb, stream);
+  rc = (key && par && sp && sc) ? FMB_OK : fmb_set_error(FMB_ERR_NOMEM, "octant pairs");
+  if (!rc) rc = fmbh_d2h(key, t->d_keys[l], sizeof(int) * nb, stream);
   if (!rc) rc = fmbh_d2h(par, t->d_parent[l], sizeof(int) * nb, stream);
   if (!rc) {
     for (int i = 0; i < nb; i++) start[(key[i] & 7) + 1]++;
