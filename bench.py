@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="fixed tile-shape rule instead of timing the shapes at plan creation")
+    ap.add_argument("--split-bf16", action="store_true",
+                    help="split-BF16 rotations in the FP32 pass's LDS-mode translations")
     ap.add_argument("--serial-levels", action="store_true",
                     help="all translations on one stream (per-level times without overlap)")
     a = ap.parse_args()
@@ -158,7 +160,8 @@ def main():
     tree = F.Tree(mesh, depth=w["depth"])
     plan = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=w["nrhs"],
                   autotune=0 if a.no_autotune else 1,
-                  serial_levels=1 if a.serial_levels else 0)
+                  serial_levels=1 if a.serial_levels else 0,
+                  split_bf16=1 if a.split_bf16 else 0)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     log("setup %.2fs (autotune %s) orders %s mem %.2f GB"
@@ -248,12 +251,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (mixed: FP32 translations/near field, FP64 recurrences and Arnoldi)",
+        "dtype": ("fp32 (mixed: split-BF16 rotations with FP32 accumulation in the LDS-mode "
+                  "translations, FP32 elsewhere; FP64 recurrences and Arnoldi)" if a.split_bf16 else
+                  "fp32 (mixed: FP32 translations/near field, FP64 recurrences and Arnoldi)"),
         "data": "synthetic icosphere and plane waves generated in-process",
         "config": {"workload": a.workload, "triangles": int(n), "freq_hz": w["freq"],
                    "radius_m": w["radius"], "depth": w["depth"], "nrhs": nrhs,
                    "orders": [int(q) for q in plan.orders()[2:w["depth"] + 1]],
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "translations": "split-bf16 rotations" if a.split_bf16 else "fp32"},
         "fp64_matvecs_per_s": mv64,
         "speedup_mixed_vs_fp64_matvec": mv32 / mv64,
         "matvec_rel_err_mixed_vs_fp64": err_mv,
@@ -274,7 +280,7 @@ def main():
                                    "fp64": flops_mv / dt64 * a.steps / 1e12},
         "cpu_baseline": cpu,
         "tile_shapes": {
-            name: {str(l): list(plan.tile_shape(0, l, prec)[:4]) for l in range(2, w["depth"] + 1)}
+            name: {str(l): list(plan.tile_shape_bf(0, l, prec)[:5]) for l in range(2, w["depth"] + 1)}
             for name, prec in (("m2l_fp32", F.FMB_MIXED), ("m2l_fp64", F.FMB_FP64))},
     }
     if solve:

@@ -51,6 +51,7 @@ class PlanParams(ctypes.Structure):
         ("autotune", ctypes.c_int),
         ("serial_levels", ctypes.c_int),
         ("waves", ctypes.c_int),
+        ("split_bf16", ctypes.c_int),
     ]
 
 
@@ -120,7 +121,7 @@ _SIGS = {
     "fmb_gmres_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "fmb_plan_tile_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                           _vp, _vp, _vp, _vp, _vp]),
+                                           _vp, _vp, _vp, _vp, _vp, _vp]),
     "fmb_plan_wavenumber": (ctypes.c_int, [_vp, _vp]),
     "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -339,12 +340,18 @@ class Plan:
     def tile_shape(self, kind, level, prec=FMB_FP64):
         """(mode, nt, mt, waves per workgroup, tuned ms) of a matrix-core
         translation (fmb_plan_tile_shape; kind 0 M2L, 1 M2M, 2 L2L)."""
-        v = [ctypes.c_int() for _ in range(4)]
+        r = self.tile_shape_bf(kind, level, prec)
+        return r[:4] + (r[5],)
+
+    def tile_shape_bf(self, kind, level, prec=FMB_FP64):
+        """(mode, nt, mt, waves per workgroup, split-BF16 rotations, tuned ms)."""
+        v = [ctypes.c_int() for _ in range(5)]
         ms = ctypes.c_double()
         _check(lib().fmb_plan_tile_shape(self.h, kind, level, prec, ctypes.byref(v[0]),
                                          ctypes.byref(v[1]), ctypes.byref(v[2]),
-                                         ctypes.byref(v[3]), ctypes.byref(ms)))
-        return v[0].value, v[1].value, v[2].value, v[3].value, ms.value
+                                         ctypes.byref(v[3]), ctypes.byref(v[4]),
+                                         ctypes.byref(ms)))
+        return tuple(x.value for x in v) + (ms.value,)
 
     def orders(self):
         p = np.zeros(16, dtype=np.int32)

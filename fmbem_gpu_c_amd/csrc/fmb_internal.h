@@ -71,6 +71,9 @@ typedef struct fmb_opset {
   void *d_phase[3];  /* e^{i m phi}, m < Pmax: M2L offsets, M2M, L2L octants */
   void *d_phase_c[3]; /* e^{-i m phi}: the same buckets, transposed pass    */
   size_t hpad_stride;
+  void *d_Hb;        /* FP32 pass with split_bf16: rotation blocks as bf16
+                        hi and lo parts [nbeta][2][hb_half]                 */
+  size_t hb_half;
   int gws_grid;      /* persistent workgroups sharing d_gws                  */
 } fmb_opset;
 
@@ -82,6 +85,7 @@ typedef struct fmb_rcr_tab {
   int pin, pout, P;     /* source order, target order, coaxial order       */
   int mode[2], nt[2], mt[2]; /* per opset: 0 = VALU kernel, else FMBH_RCR_* */
   int nw[2];            /* per opset: waves per workgroup (4 or 8)         */
+  int bf[2];            /* per opset: split-BF16 rotations                 */
   int *d_sched[2];      /* per opset: unit schedule                        */
   int sched_len[2];     /* ints in d_sched                                 */
   void *d_Cpad[2];      /* per opset: zero-padded coaxial blocks           */

@@ -85,6 +85,9 @@ int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 #define FMBH_RCR_LDSP 3 /* as LDS, a persistent grid looping over batches */
 typedef struct fmbh_rcr_job {
   int mode, nt, mt, nw; /* nw: waves per workgroup, 4 or 8 */
+  int bf;               /* split-BF16 rotations (FP32 pass, LDS modes)   */
+  const void *Hb;       /* their operator blocks, per beta: hi then lo    */
+  size_t hbstride, hbhalf;
   const void *chunks;
   int nchunks, chunk_pairs;
   const void *info;
@@ -116,6 +119,11 @@ int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
  * (rows pout-|m|, columns pin-|m|) at coff[|m|] */
 int fmbh_pad_rot(int prec, const void *d_H, int nbeta, int Pmax,
                  const int *d_hoff, size_t pstride, void *d_Hp, void *stream);
+/* split-BF16 copies of the FP32 padded rotation blocks: block n of beta b
+ * at hboff[n] (rows padded to 16, K to 32, row-major), hi part at
+ * b * 2 half, lo part half values further */
+int fmbh_split_rot(const void *d_Hp, size_t pstride, const int *d_hoff, int nbeta, int Pmax,
+                   const int *d_hboff, size_t half, void *d_Hb, void *stream);
 int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pin,
                   int pout, const int *d_coff, size_t pstride, void *d_Cp,
                   void *stream);

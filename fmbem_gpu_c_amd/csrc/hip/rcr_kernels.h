@@ -46,6 +46,8 @@ int run_f32_w4(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
 int run_f32_w8(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
 int run_f64_w4(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
 int run_f64_w8(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
+int run_bf_w4(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
+int run_bf_w8(const fmbh_rcr_job *j, int nrhs, hipStream_t st);
 }  // namespace fmbh_rcr_detail
 
 namespace {
@@ -99,6 +101,8 @@ struct RcrArgs {
   size_t hstride;
   const void *C;       // padded coaxial blocks, per coaxial matrix
   size_t cstride;
+  const void *Hb;      // split-BF16 rotation blocks, per beta: hi, then lo
+  size_t hbstride, hbhalf;
   int P, pin, pout;
   const int *sched;    // offsets [3][W + 1], then {code, operator offset}
   int sched_len;
@@ -324,6 +328,121 @@ __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
   }
 }
 
+// ---- split-BF16 rotations (FP32 batches, FP32 accumulation) ----
+// v_mfma_f32_16x16x32_bf16: A[i][k] in lane i + 16 (k / 8), element k % 8,
+// and B[k][j] in lane j + 16 (k / 8), element k % 8; D as in the FP32 form.
+// Each operand is a sum hi + lo of two bf16 values, and a product takes the
+// three terms hi.hi + hi.lo + lo.hi (relative error about 2^-16).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Row tiles [t0, t0 + NTL) of rotation block n (size s, first row base) in
+// split-BF16: Ab is the block's hi part (rows padded to 16, K to 32,
+// row-major); its lo part lies `half` values further on.
+template <int NT, int NTL>
+__device__ __forceinline__ void rot_unit_bf(const float *Xi, float *Xo,
+                                            const __bf16 *__restrict__ Ab, size_t half,
+                                            int s, int base, int t0) {
+  using acc_t = Mfma<float>::acc_t;
+  constexpr int RS = Row<NT>::stride;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int kp = (s + 31) & ~31, nks = kp >> 5;
+  const __bf16 *ap = Ab + (size_t)(16 * t0 + li) * kp + 8 * lk;
+  acc_t acc[NTL][NT];
+  for (int ks = 0; ks < nks; ks++) {
+    bf16x8 ahi[NTL], alo[NTL], bhi[NT], blo[NT];
+#pragma unroll
+    for (int mt = 0; mt < NTL; mt++) {
+      const __bf16 *a = ap + (size_t)16 * mt * kp + 32 * ks;
+      ahi[mt] = *(const bf16x8 *)a;
+      alo[mt] = *(const bf16x8 *)(a + half);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      // rows past the block meet zero operator columns; they are read at a
+      // clamped index so that they stay finite
+      const int k = min(32 * ks + 8 * lk + e, s - 1);
+      const float *xr = Xi + (base + k) * RS + li;
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const float x = xr[16 * t];
+        const __bf16 h = (__bf16)x;
+        bhi[t][e] = h;
+        blo[t][e] = (__bf16)(x - (float)h);
+      }
+    }
+    if (ks == 0) {
+#pragma unroll
+      for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt], bhi[t],
+                                                               acc_t{0, 0, 0, 0}, 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt], bhi[t], acc[mt][t],
+                                                               0, 0, 0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < NTL; mt++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt], blo[t], acc[mt][t], 0, 0, 0);
+        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[mt], bhi[t], acc[mt][t], 0, 0, 0);
+      }
+  }
+  // rows of a full tile are stored unguarded (a wave-uniform test)
+#pragma unroll
+  for (int mt = 0; mt < NTL; mt++) {
+    const int r0 = 16 * (t0 + mt);
+    float *xo = Xo + (base + r0) * RS + li;
+    if (r0 + 16 <= s) {
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) xo[Mfma<float>::row(lk, e) * RS + 16 * t] = acc[mt][t][e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int i = Mfma<float>::row(lk, e);
+        if (r0 + i < s) {
+#pragma unroll
+          for (int t = 0; t < NT; t++) xo[i * RS + 16 * t] = acc[mt][t][e];
+        }
+      }
+    }
+  }
+}
+
+template <int NT, int MT>
+__device__ __forceinline__ void rot_any_bf(int ntl, const float *Xi, float *Xo,
+                                           const __bf16 *Ab, size_t half, int s, int base,
+                                           int t0) {
+  if constexpr (MT > 1) {
+    if (ntl < MT) {
+      rot_any_bf<NT, MT - 1>(ntl, Xi, Xo, Ab, half, s, base, t0);
+      return;
+    }
+  }
+  rot_unit_bf<NT, MT>(Xi, Xo, Ab, half, s, base, t0);
+}
+
+// the rotation stage in split-BF16; unit operator offsets index the split
+// blocks
+template <int NT, int MT>
+__device__ __forceinline__ void rot_stage_bf(const float *Xi, float *Xo, const __bf16 *Hb,
+                                             size_t half, const int *units, int count) {
+  for (int j = 0; j < count; j++) {
+    const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
+    const int n = code >> 4, t0 = (code & 15) * MT;
+    const int s = 2 * n + 1;
+    const int ntl = min(MT, ((s + 15) >> 4) - t0);
+    rot_any_bf<NT, MT>(ntl, Xi, Xo, Hb + aoff, half, s, n * n, t0);
+  }
+}
+
 // Coaxial stage over this wave's units {am*16 + g, operator offset}: the
 // blocks +am and -am together.
 template <typename T, int NT, int MT>
@@ -342,7 +461,7 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
 }
 
 // One batch: vectors u0 .. u0+V-1 of chunk ch.
-template <typename T, int NT, int MT, int W, bool GWS>
+template <typename T, int NT, int MT, int W, bool GWS, bool BF = false>
 __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
                                           T *X0, T *X1, cplx<T> *ph,
                                           const int *sched) {
@@ -393,11 +512,20 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   const int *u1 = sched + 2 * uni(o1[wave]), w1 = uni(o1[wave + 1] - o1[wave]);
   const int *uc = sched + 2 * uni(oc[wave]), wc = uni(oc[wave + 1] - oc[wave]);
   const int *u3 = sched + 2 * uni(o3[wave]), w3 = uni(o3[wave + 1] - o3[wave]);
-  rot_stage<T, NT, MT>(X0, X1, H, u1, w1);
-  __syncthreads();
-  coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
-  __syncthreads();
-  rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
+  if constexpr (BF) {
+    const __bf16 *Hb = (const __bf16 *)a.Hb + a.hbstride * oi.beta;
+    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u1, w1);
+    __syncthreads();
+    coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
+    __syncthreads();
+    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u3, w3);
+  } else {
+    rot_stage<T, NT, MT>(X0, X1, H, u1, w1);
+    __syncthreads();
+    coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
+    __syncthreads();
+    rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
+  }
   __syncthreads();
   // add e^{-i m' phi} y into the targets.  Thread e adds real e of each
   // target vector (part e & 1 of coefficient e >> 1), so that one wave's
@@ -424,7 +552,7 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
 template <typename T, int W> constexpr int kWavesPerEu = sizeof(T) == 4 ? (W == 8 ? 4 : 3) : 1;
 
 // LDS mode: grid (nchunks, nb), one batch per workgroup.
-template <typename T, int NT, int MT, int W>
+template <typename T, int NT, int MT, int W, bool BF = false>
 __global__ void __launch_bounds__(kThreads<W>)
     __attribute__((amdgpu_waves_per_eu(kWavesPerEu<T, W>))) k_rcr_lds(RcrArgs a) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
@@ -437,13 +565,13 @@ __global__ void __launch_bounds__(kThreads<W>)
   cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * RS);
   int *sched = (int *)(ph + Pm + (Pm & 1));
   for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
-  rcr_batch<T, NT, MT, W, false>(a, ch, u0, X, X, ph, sched);
+  rcr_batch<T, NT, MT, W, false, BF>(a, ch, u0, X, X, ph, sched);
 }
 
 // LDS mode on a persistent grid (one workgroup per free slot of the GPU):
 // each workgroup loads the schedule once and loops over the items chunk * nb
 // + batch, so workgroups are not dispatched (and set up) per batch.
-template <typename T, int NT, int MT, int W>
+template <typename T, int NT, int MT, int W, bool BF = false>
 __global__ void __launch_bounds__(kThreads<W>)
     __attribute__((amdgpu_waves_per_eu(kWavesPerEu<T, W>))) k_rcr_ldsp(RcrArgs a) {
   constexpr int V = 8 * NT;
@@ -458,7 +586,7 @@ __global__ void __launch_bounds__(kThreads<W>)
     const int c = (int)(it / a.nb), b = (int)(it % a.nb);
     const int4 ch = a.chunks[c];
     if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
-    rcr_batch<T, NT, MT, W, false>(a, ch, b * V, X, X, ph, sched);
+    rcr_batch<T, NT, MT, W, false, BF>(a, ch, b * V, X, X, ph, sched);
   }
 }
 
@@ -515,7 +643,8 @@ int set_lds(K kern, size_t lds) {
   return FMB_OK;
 }
 
-template <typename T, int NT, int MT, int W>
+// BF: split-BF16 rotations (FP32 batches in LDS mode only)
+template <typename T, int NT, int MT, int W, bool BF = false>
 int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   constexpr int V = 8 * NT;
   const int Pm = j->pin > j->pout ? j->pin : j->pout;
@@ -532,6 +661,9 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   a.hstride = j->hstride;
   a.C = j->C;
   a.cstride = j->cstride;
+  a.Hb = j->Hb;
+  a.hbstride = j->hbstride;
+  a.hbhalf = j->hbhalf;
   a.P = j->P;
   a.pin = j->pin;
   a.pout = j->pout;
@@ -544,13 +676,13 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   const size_t tsz = sizeof(T);
   if (j->mode == FMBH_RCR_LDS) {
     const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
-    auto kern = k_rcr_lds<T, NT, MT, W>;
+    auto kern = k_rcr_lds<T, NT, MT, W, BF>;
     int rc = set_lds(kern, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(kern, dim3(a.nchunks, a.nb), dim3(kThreads<W>), lds, st, a);
   } else if (j->mode == FMBH_RCR_LDSP) {
     const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
-    auto kern = k_rcr_ldsp<T, NT, MT, W>;
+    auto kern = k_rcr_ldsp<T, NT, MT, W, BF>;
     int rc = set_lds(kern, lds);
     if (rc) return rc;
     int per_cu = 0, ncu = 0, dev = 0;
@@ -561,6 +693,8 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
     long long items = (long long)a.nchunks * a.nb, cap = (long long)per_cu * ncu;
     int grid = (int)(items < cap ? items : cap);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads<W>), lds, st, a);
+  } else if constexpr (BF) {
+    return fmb_set_error(FMB_ERR_RANGE, "split-BF16 translations need an LDS tile shape");
   } else {
     const size_t lds = small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_gws<T, NT, MT, W>;
@@ -575,10 +709,10 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
 }
 
 // the tile shapes (nt, mt) of one precision and wave count
-template <typename T, int W>
+template <typename T, int W, bool BF = false>
 int dispatch_shape(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
 #define CASE(A, B) \
-  if (j->nt == A && j->mt == B) return launch<T, A, B, W>(j, nrhs, st);
+  if (j->nt == A && j->mt == B) return launch<T, A, B, W, BF>(j, nrhs, st);
   CASE(1, 2) CASE(1, 3) CASE(1, 4) CASE(1, 6)
   CASE(2, 2) CASE(2, 3) CASE(2, 4) CASE(2, 6)
 #undef CASE

@@ -19,6 +19,24 @@ __global__ void k_pad_rot(const T *__restrict__ H, size_t hs, const int *__restr
   }
 }
 
+// split-BF16 rotation blocks from the FP32 padded ones (column-major,
+// leading dimension pad16(s)): row-major, K padded to 32, hi = bf16(h),
+// lo = bf16(h - hi)
+__global__ void k_split_rot(const float *__restrict__ Hp, size_t ps, const int *__restrict__ hoff,
+                            const int *__restrict__ hboff, size_t half, __bf16 *__restrict__ Hb) {
+  const int n = blockIdx.x, b = blockIdx.y;
+  const int s = 2 * n + 1, ld = (s + 15) & ~15, kp = (s + 31) & ~31;
+  const float *src = Hp + ps * b + hoff[n];
+  __bf16 *hi = Hb + 2 * half * b + hboff[n];
+  for (int e = threadIdx.x; e < ld * kp; e += blockDim.x) {
+    const int r = e / kp, k = e % kp;
+    const float v = (r < s && k < s) ? src[k * ld + r] : 0.0f;
+    const __bf16 h = (__bf16)v;
+    hi[e] = h;
+    hi[e + half] = (__bf16)(v - (float)h);
+  }
+}
+
 template <typename T>
 __global__ void k_pad_coax(const cplx<T> *__restrict__ C, size_t cs, int P, int pin,
                            int pout, const int *__restrict__ coff, size_t ps,
@@ -75,6 +93,11 @@ extern "C" int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream)
   if (j->nw != 4 && j->nw != 8)
     return fmb_set_error(FMB_ERR_RANGE, "no RCR tile shape with %d waves", j->nw);
   hipStream_t st = (hipStream_t)stream;
+  if (j->bf) {
+    if (prec != FMB_MIXED)
+      return fmb_set_error(FMB_ERR_ARG, "split-BF16 translations are for the FP32 pass");
+    return j->nw == 8 ? run_bf_w8(j, nrhs, st) : run_bf_w4(j, nrhs, st);
+  }
   if (prec == FMB_FP64) return j->nw == 8 ? run_f64_w8(j, nrhs, st) : run_f64_w4(j, nrhs, st);
   return j->nw == 8 ? run_f32_w8(j, nrhs, st) : run_f32_w4(j, nrhs, st);
 }
@@ -108,6 +131,15 @@ extern "C" int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pi
     hipLaunchKernelGGL(k_pad_coax<float>, dim3(mn, ncoax), dim3(256), 0, st,
                        (const cplx<float> *)d_C, cs, P, pin, pout, d_coff, pstride,
                        (cplx<float> *)d_Cp);
+  FMB_LAUNCH_CHECK();
+  return FMB_OK;
+}
+
+extern "C" int fmbh_split_rot(const void *d_Hp, size_t pstride, const int *d_hoff, int nbeta,
+                              int Pmax, const int *d_hboff, size_t half, void *d_Hb,
+                              void *stream) {
+  hipLaunchKernelGGL(k_split_rot, dim3(Pmax, nbeta), dim3(256), 0, (hipStream_t)stream,
+                     (const float *)d_Hp, pstride, d_hoff, d_hboff, half, (__bf16 *)d_Hb);
   FMB_LAUNCH_CHECK();
   return FMB_OK;
 }

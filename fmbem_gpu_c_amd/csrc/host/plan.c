@@ -140,7 +140,9 @@ static void opset_free_freq(fmb_opset *o) {
 static void opset_free_rot(fmb_opset *o) {
   fmbh_free(o->d_H);
   fmbh_free(o->d_Hpad);
-  o->d_H = o->d_Hpad = NULL;
+  fmbh_free(o->d_Hb);
+  o->d_H = o->d_Hpad = o->d_Hb = NULL;
+  o->hb_half = 0;
   for (int i = 0; i < 3; i++) {
     fmbh_free(o->d_phase[i]);
     fmbh_free(o->d_phase_c[i]);
@@ -499,6 +501,9 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
   if (prm->waves != 0 && prm->waves != 4 && prm->waves != 8)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: waves %d (0, 4 or 8)", prm->waves);
+  if (prm->split_bf16 && (prm->precision != FMB_MIXED || prm->rcr_path != FMB_RCR_AUTO))
+    return fmb_set_error(FMB_ERR_ARG,
+                         "fmb_plan_create: split_bf16 needs an FMB_MIXED plan on the matrix cores");
   int rc;
   fmb_plan *pl = (fmb_plan *)calloc(1, sizeof(fmb_plan));
   if (!pl) return fmb_set_error(FMB_ERR_NOMEM, "fmb_plan_create");

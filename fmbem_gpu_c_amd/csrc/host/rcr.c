@@ -15,6 +15,7 @@
  * workgroup's waves by longest-processing-time first. */
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -83,6 +84,19 @@ static int pad_cols(int s) { return 4 * ksteps(s); }
 static void rot_offsets(int np, int *hoff) {
   hoff[0] = 0;
   for (int n = 0; n < np; n++) hoff[n + 1] = hoff[n] + pad_rows(2 * n + 1) * pad_cols(2 * n + 1);
+}
+
+/* offsets of the split-BF16 rotation blocks n < np (np + 1 values): rows
+ * padded to 16, K to 32 */
+static void rot_offsets_bf(int np, int *hb) {
+  hb[0] = 0;
+  for (int n = 0; n < np; n++) hb[n + 1] = hb[n] + pad_rows(2 * n + 1) * (((2 * n + 1) + 31) & ~31);
+}
+
+/* whether opset op may run mode's rotations in split-BF16 */
+static int use_bf(const fmb_plan *pl, int op, int mode) {
+  return pl->prm.split_bf16 && pl->ops[op].prec != FMB_FP64 &&
+         (mode == FMBH_RCR_LDS || mode == FMBH_RCR_LDSP);
 }
 
 /* offsets of the padded coaxial blocks |m| < min(pin, pout) (+1 values) */
@@ -219,13 +233,15 @@ static int octant_pairs(const fmb_tree *t, int l, fmb_rcr_tab *tb, void *stream)
 /* tile shapes, schedules and padded coaxial blocks of one translation;
  * C: the scaled coaxial matrices of each opset (ncoax of them, order P) */
 static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
-                    void *const *C, int ncoax, const int *hoff, void *stream) {
+                    void *const *C, int ncoax, const int *hoff, const int *hboff,
+                    void *stream) {
   int rc = FMB_OK, mn = pin < pout ? pin : pout;
   tb->pin = pin;
   tb->pout = pout;
   tb->P = pin > pout ? pin : pout;
   int *coff = (int *)malloc(sizeof(int) * (mn + 1));
   int *d_coff = NULL;
+  if (!coff) return fmb_set_error(FMB_ERR_NOMEM, "coaxial offsets");
   coax_offsets(pin, pout, coff);
   tb->cpad_stride = (size_t)coff[mn];
   for (int op = 0; op < pl->nops && !rc; op++) {
@@ -235,8 +251,10 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
     if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
       break;
     tb->nw[op] = pl->prm.waves ? pl->prm.waves : 4;
-    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op], hoff, coff,
-                          &tb->d_sched[op], &tb->sched_len[op], stream)))
+    tb->bf[op] = use_bf(pl, op, tb->mode[op]);
+    if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op],
+                          tb->bf[op] ? hboff : hoff, coff, &tb->d_sched[op],
+                          &tb->sched_len[op], stream)))
       break;
     if (!d_coff) {
       if ((rc = fmbh_malloc((void **)&d_coff, sizeof(int) * (mn + 1)))) break;
@@ -309,6 +327,27 @@ int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
       rc = fmbh_pad_rot(o->prec, o->d_H, pl->nbeta, pl->Pmax, d_hoff, o->hpad_stride,
                         o->d_Hpad, stream);
     }
+    /* split-BF16 copies for the FP32 pass */
+    if (!rc && use_bf(pl, op, FMBH_RCR_LDS)) {
+      int *hb = (int *)malloc(sizeof(int) * (pl->Pmax + 1)), *d_hb = NULL;
+      if (!hb) rc = fmb_set_error(FMB_ERR_NOMEM, "split-BF16 offsets");
+      if (!rc) {
+        rot_offsets_bf(pl->Pmax, hb);
+        o->hb_half = (size_t)hb[pl->Pmax];
+        size_t hbytes = 2 * sizeof(uint16_t) * o->hb_half * pl->nbeta;
+        rc = fmbh_malloc((void **)&d_hb, sizeof(int) * (pl->Pmax + 1));
+        if (!rc) rc = fmbh_h2d(d_hb, hb, sizeof(int) * (pl->Pmax + 1), stream);
+        if (!rc) rc = fmbh_malloc(&o->d_Hb, hbytes);
+        if (!rc) {
+          pl->bytes += hbytes;
+          rc = fmbh_split_rot(o->d_Hpad, o->hpad_stride, d_hoff, pl->nbeta, pl->Pmax, d_hb,
+                              o->hb_half, o->d_Hb, stream);
+        }
+        if (!rc) rc = fmbh_sync(stream);
+      }
+      fmbh_free(d_hb);
+      free(hb);
+    }
   }
   if (!rc) rc = fmbh_sync(stream);
   fmbh_free(d_hoff);
@@ -352,26 +391,29 @@ static int size_gws(fmb_plan *pl, int op, size_t min_per_wg) {
  * box pairs and buffers at nrhs_max right-hand sides, in every valid shape:
  * LDS mode with 8 or 16 vectors per batch (when the batch fits the 160 KB of
  * LDS), and global-workspace mode with 8 or 16 vectors and row groups of
- * 2, 3, 4 or 6 tiles, each with 4 or 8 waves per workgroup.  The fastest
+ * 2, 3, 4 or 6 tiles, each with 4 or 8 waves per workgroup, and with
+ * split_bf16 the LDS shapes once more with split-BF16 rotations.  The fastest
  * shape (a one-vector warm-up, then one timed run, HIP events on the plan's
  * stream) replaces the fixed rule of fmbh_rcr_shape. */
 typedef struct {
-  int mode, nt, mt, nw;
+  int mode, nt, mt, nw, bf;
 } shape_t;
 
-static int shape_candidates(int P, int prec, shape_t *c) {
+/* bfok: LDS shapes are also tried with split-BF16 rotations */
+static int shape_candidates(int P, int bfok, shape_t *c) {
   int n = 0, mt = fmbh_rcr_mt_lds(P);
   static const int mts[4] = {2, 3, 4, 6};
   for (int nw = 4; nw <= 8; nw += 4) {
-    for (int nt = 1; nt <= 2; nt++)
-      if (mt) {
-        shape_t v = {FMBH_RCR_LDS, nt, mt, nw}, w = {FMBH_RCR_LDSP, nt, mt, nw};
-        c[n++] = v;
-        c[n++] = w;
-      }
+    for (int bf = 0; bf <= bfok; bf++)
+      for (int nt = 1; nt <= 2; nt++)
+        if (mt) {
+          shape_t v = {FMBH_RCR_LDS, nt, mt, nw, bf}, w = {FMBH_RCR_LDSP, nt, mt, nw, bf};
+          c[n++] = v;
+          c[n++] = w;
+        }
     for (int nt = 1; nt <= 2; nt++)
       for (int i = 0; i < 4; i++) {
-        shape_t v = {FMBH_RCR_GWS, nt, mts[i], nw};
+        shape_t v = {FMBH_RCR_GWS, nt, mts[i], nw, 0};
         c[n++] = v;
       }
   }
@@ -399,7 +441,7 @@ static int time_run(fmb_plan *pl, int op, int kind, int l, void *stream, float *
  * on: precision, kind, orders, chunk count and right-hand sides (a sweep
  * that revisits a level's order reuses its result). */
 typedef struct {
-  int prec, kind, pin, pout, nchunks, nrhs;
+  int prec, kind, pin, pout, nchunks, nrhs, bf;
   shape_t shape;
   float ms;
 } tuned_t;
@@ -414,7 +456,8 @@ static int cache_find(const tuned_t *key, tuned_t *out) {
   for (int i = 0; i < tune_cached && !hit; i++) {
     const tuned_t *c = &tune_cache[i];
     if (c->prec == key->prec && c->kind == key->kind && c->pin == key->pin &&
-        c->pout == key->pout && c->nchunks == key->nchunks && c->nrhs == key->nrhs) {
+        c->pout == key->pout && c->nchunks == key->nchunks && c->nrhs == key->nrhs &&
+        c->bf == key->bf) {
       *out = *c;
       hit = 1;
     }
@@ -440,7 +483,14 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
     if (!rc) rc = fmbh_memset(o->d_L[l], 0, bytes, stream);
   }
   int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
+  int *hboff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
+  if (!hoff || !hboff) {
+    free(hoff);
+    free(hboff);
+    return fmb_set_error(FMB_ERR_NOMEM, "rotation offsets");
+  }
   rot_offsets(pl->Pmax, hoff);
+  rot_offsets_bf(pl->Pmax, hboff);
   for (int kind = 0; kind < 3 && !rc; kind++)
     for (int l = (kind ? 3 : 2); l <= L && !rc; l++) {
       fmb_rcr_tab *tb = &pl->rcr[kind][l];
@@ -448,10 +498,14 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
       if (!tb->mode[op] || !pairs->nchunks) continue;
       int mn = tb->pin < tb->pout ? tb->pin : tb->pout;
       int *coff = (int *)malloc(sizeof(int) * (mn + 1));
+      if (!coff) {
+        rc = fmb_set_error(FMB_ERR_NOMEM, "coaxial offsets");
+        break;
+      }
       coax_offsets(tb->pin, tb->pout, coff);
       tuned_t key = {o->prec, kind, tb->pin, tb->pout, pairs->nchunks, pl->nrhs_max,
-                     {0, 0, 0, 0}, 0.0f}, hit;
-      shape_t c[32];
+                     use_bf(pl, op, FMBH_RCR_LDS), {0, 0, 0, 0, 0}, 0.0f}, hit;
+      shape_t c[40];
       int nc;
       float best = 0.0f;
       if (cache_find(&key, &hit)) {
@@ -461,15 +515,16 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         best = 1e30f;
       } else {
         rc = time_run(pl, op, kind, l, stream, &best);
-        nc = shape_candidates(tb->P, o->prec, c);
+        nc = shape_candidates(tb->P, use_bf(pl, op, FMBH_RCR_LDS), c);
       }
       for (int i = 0; i < nc && !rc; i++) {
         if (c[i].mode == tb->mode[op] && c[i].nt == tb->nt[op] && c[i].mt == tb->mt[op] &&
-            c[i].nw == tb->nw[op])
+            c[i].nw == tb->nw[op] && c[i].bf == tb->bf[op])
           continue;
         int *sched = NULL, slen = 0;
-        if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, c[i].nw, hoff,
-                              coff, &sched, &slen, stream)))
+        const int bf = c[i].bf;
+        if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, c[i].nw,
+                              bf ? hboff : hoff, coff, &sched, &slen, stream)))
           break;
         if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP) {
           size_t b = fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
@@ -481,12 +536,13 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
           fmbh_free(sched);
           break;
         }
-        shape_t prev = {tb->mode[op], tb->nt[op], tb->mt[op], tb->nw[op]};
+        shape_t prev = {tb->mode[op], tb->nt[op], tb->mt[op], tb->nw[op], tb->bf[op]};
         int *psched = tb->d_sched[op], plen = tb->sched_len[op];
         tb->mode[op] = c[i].mode;
         tb->nt[op] = c[i].nt;
         tb->mt[op] = c[i].mt;
         tb->nw[op] = c[i].nw;
+        tb->bf[op] = bf;
         tb->d_sched[op] = sched;
         tb->sched_len[op] = slen;
         float ms;
@@ -499,6 +555,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
           tb->nt[op] = prev.nt;
           tb->mt[op] = prev.mt;
           tb->nw[op] = prev.nw;
+          tb->bf[op] = prev.bf;
           tb->d_sched[op] = psched;
           tb->sched_len[op] = plen;
           fmbh_free(sched);
@@ -511,12 +568,14 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         key.shape.nt = tb->nt[op];
         key.shape.mt = tb->mt[op];
         key.shape.nw = tb->nw[op];
+        key.shape.bf = tb->bf[op];
         key.ms = best;
         if (!cache_find(&key, &hit)) cache_put(&key);
       }
       free(coff);
     }
   free(hoff);
+  free(hboff);
   if (!rc) rc = size_gws(pl, op, 0);
   return rc;
 }
@@ -537,22 +596,30 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
   }
   if (pl->prm.rcr_path != FMB_RCR_VALU) {
     int *hoff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
+    int *hboff = (int *)malloc(sizeof(int) * (pl->Pmax + 1));
+    if (!hoff || !hboff) {
+      free(hoff);
+      free(hboff);
+      return fmb_set_error(FMB_ERR_NOMEM, "rotation offsets");
+    }
     rot_offsets(pl->Pmax, hoff);
+    rot_offsets_bf(pl->Pmax, hboff);
     for (int l = 2; l <= L && !rc; l++) {
       fmb_rcr_tab *tb = &pl->rcr[0][l];
       void *C[2] = {pl->ops[0].d_Csr[l], pl->ops[1].d_Csr[l]};
       if ((rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream))) break;
-      rc = plan_tab(pl, tb, pl->p[l], pl->p[l], C, FMB_NDIST, hoff, stream);
+      rc = plan_tab(pl, tb, pl->p[l], pl->p[l], C, FMB_NDIST, hoff, hboff, stream);
     }
     for (int l = 3; l <= L && !rc; l++) {
       fmb_rcr_tab *up = &pl->rcr[1][l], *dn = &pl->rcr[2][l];
       void *Cu[2] = {pl->ops[0].d_Cm2m[l], pl->ops[1].d_Cm2m[l]};
       void *Cd[2] = {pl->ops[0].d_Cl2l[l], pl->ops[1].d_Cl2l[l]};
       if ((rc = octant_pairs(t, l, up, stream))) break;
-      if ((rc = plan_tab(pl, up, pl->p[l], pl->p[l - 1], Cu, 1, hoff, stream))) break;
-      rc = plan_tab(pl, dn, pl->p[l - 1], pl->p[l], Cd, 1, hoff, stream);
+      if ((rc = plan_tab(pl, up, pl->p[l], pl->p[l - 1], Cu, 1, hoff, hboff, stream))) break;
+      rc = plan_tab(pl, dn, pl->p[l - 1], pl->p[l], Cd, 1, hoff, hboff, stream);
     }
     free(hoff);
+    free(hboff);
     if (rc) return rc;
   }
   for (int op = 0; op < pl->nops; op++)
@@ -600,6 +667,10 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.nt = tb->nt[op];
   j.mt = tb->mt[op];
   j.nw = tb->nw[op];
+  j.bf = tb->bf[op];
+  j.Hb = o->d_Hb;
+  j.hbstride = 2 * o->hb_half;
+  j.hbhalf = o->hb_half;
   j.chunks = pairs->d_chunks;
   j.nchunks = pairs->nchunks;
   j.chunk_pairs = RCR_CHUNK;
@@ -664,6 +735,10 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.nt = tb->nt[op];
   j.mt = tb->mt[op];
   j.nw = tb->nw[op];
+  j.bf = tb->bf[op];
+  j.Hb = o->d_Hb;
+  j.hbstride = 2 * o->hb_half;
+  j.hbhalf = o->hb_half;
   j.chunks = pairs->d_chunks;
   j.nchunks = pairs->nchunks;
   j.chunk_pairs = RCR_CHUNK;
@@ -703,7 +778,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
 }
 
 int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
-                        int *nt, int *mt, int *nw, double *ms) {
+                        int *nt, int *mt, int *nw, int *bf, double *ms) {
   if (!pl || kind < 0 || kind > 2 || level < 2 || level > pl->tree->lmax ||
       (kind && level < 3))
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_tile_shape: invalid arguments");
@@ -713,6 +788,7 @@ int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *
   if (nt) *nt = tb->nt[op];
   if (mt) *mt = tb->mt[op];
   if (nw) *nw = tb->nw[op];
+  if (bf) *bf = tb->bf[op];
   if (ms) *ms = tb->tuned_ms[op];
   return FMB_OK;
 }

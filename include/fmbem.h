@@ -165,6 +165,11 @@ typedef struct fmb_plan_params {
   int waves;                 /* waves per workgroup of the matrix-core
                                 translations in the fixed tile-shape rule:
                                 4 or 8 (0: 4); autotune overrides it        */
+  int split_bf16;            /* FMB_MIXED plans: 1 runs the rotations of the
+                                FP32 pass's LDS-mode translations in
+                                split-BF16 (hi + lo bf16 operands, three
+                                products, FP32 accumulation; about 2^-16
+                                relative per product)                       */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);
@@ -194,10 +199,11 @@ int fmb_plan_retune(fmb_plan *pl, double k, const fmb_c64 *robin, void *stream);
  * mode (1: the batch in LDS, 2: in a global workspace, 3: in LDS on a
  * persistent grid, 0: VALU kernel), 16-column
  * tiles per batch nt (8 nt vectors), row tiles per unit mt, waves per
- * workgroup nw (4 or 8), and with autotune the timed milliseconds of one
- * launch (else 0).  Any output pointer may be NULL. */
+ * workgroup nw (4 or 8), bf = 1 where the rotations run in split-BF16, and
+ * with autotune the timed milliseconds of one launch (else 0).  Any output
+ * pointer may be NULL. */
 int fmb_plan_tile_shape(const fmb_plan *pl, int kind, int level, int prec, int *mode,
-                        int *nt, int *mt, int *nw, double *ms);
+                        int *nt, int *mt, int *nw, int *bf, double *ms);
 /* Wavenumber of the operator the plan holds (0 after a failed retune). */
 int fmb_plan_wavenumber(const fmb_plan *pl, double *k);
 /* Device bytes held by the plan. */

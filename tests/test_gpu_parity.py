@@ -342,3 +342,41 @@ def test_waves_parameter_is_checked():
     t = F.Tree(m, depth=2)
     with pytest.raises(F.FmbError):
         F.Plan(t, wavenumber(1000.0), waves=6)
+
+
+@pytest.mark.parametrize("freq", [4000.0, 12000.0])
+def test_split_bf16_rotations(freq):
+    """Split-BF16 rotations in the FP32 pass (LDS-mode translations; the
+    global-workspace ones stay FP32): the 20,480-triangle sphere at depth 6.
+    Each operand is hi + lo in bf16 and a product keeps three terms, so the
+    pass meets the FP64 one to about 1e-5 (bound 1e-4) and the FP32 pass to
+    the same; the transposed pass matches its transpose identity."""
+    m = F.Mesh.icosphere(5, 0.1)
+    t = F.Tree(m, depth=6)
+    k = wavenumber(freq)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=2)
+    p32 = F.Plan(t, k, **kw)
+    pbf = F.Plan(t, k, split_bf16=1, **kw)
+    _, L, _, _ = t.info()
+    print("f", freq, "orders", pbf.orders()[2:L + 1].tolist(),
+          "M2L modes", [pbf.tile_shape(0, l, F.FMB_MIXED)[0] for l in range(2, L + 1)], flush=True)
+    n = m.arrays()[1].shape[0]
+    rng = np.random.default_rng(47)
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    y = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    xt, yt = torch.tensor(x, device="cuda"), torch.tensor(y, device="cuda")
+    a64 = p32.matvec(xt, prec=F.FMB_FP64).cpu().numpy()
+    a32 = p32.matvec(xt, prec=F.FMB_MIXED).cpu().numpy()
+    abf = pbf.matvec(xt, prec=F.FMB_MIXED).cpu().numpy()
+    e64 = np.linalg.norm(abf - a64) / np.linalg.norm(a64)
+    e32 = np.linalg.norm(abf - a32) / np.linalg.norm(a32)
+    print("split-BF16 vs FP64 %.2e, vs FP32 %.2e (FP32 vs FP64 %.2e)"
+          % (e64, e32, np.linalg.norm(a32 - a64) / np.linalg.norm(a64)), flush=True)
+    assert e64 < 1e-4 and e32 < 1e-4
+    assert e64 > 1e-8  # the split path really ran
+    aty = pbf.matvec_t(yt, prec=F.FMB_MIXED).cpu().numpy()
+    for r in range(2):
+        lhs, rhs = np.sum(y[r] * abf[r]), np.sum(aty[r] * x[r])
+        assert abs(lhs - rhs) < 1e-4 * np.linalg.norm(y[r]) * np.linalg.norm(abf[r])
+    with pytest.raises(F.FmbError):
+        F.Plan(t, k, split_bf16=1, precision=F.FMB_FP64)
