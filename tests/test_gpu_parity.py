@@ -380,3 +380,33 @@ def test_split_bf16_rotations(freq):
         assert abs(lhs - rhs) < 1e-4 * np.linalg.norm(y[r]) * np.linalg.norm(abf[r])
     with pytest.raises(F.FmbError):
         F.Plan(t, k, split_bf16=1, precision=F.FMB_FP64)
+
+
+def test_split_bf16_retune_and_graph():
+    """A split-BF16 plan retuned to a higher frequency (orders grow, so the
+    rotation tables and their split copies are rebuilt) against a fresh
+    split-BF16 plan there (1e-6 relative: same operators, order of the
+    target sums only), and a graph-captured batch of two split-BF16 plans
+    replayed three times against their eager matvecs."""
+    m = F.Mesh.icosphere(4, 0.1)
+    t = F.Tree(m, depth=5)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=2, split_bf16=1)
+    n = m.arrays()[1].shape[0]
+    rng = np.random.default_rng(53)
+    x = torch.tensor(rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)), device="cuda")
+    pl = F.Plan(t, wavenumber(2000.0), **kw)
+    k = wavenumber(9000.0)
+    pl.retune(k)
+    fresh = F.Plan(t, k, **kw)
+    assert any(fresh.tile_shape_bf(0, l, F.FMB_MIXED)[4] for l in range(2, 6))
+    a = pl.matvec(x, prec=F.FMB_MIXED).cpu().numpy()
+    b = fresh.matvec(x, prec=F.FMB_MIXED).cpu().numpy()
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    p2 = F.Plan(t, wavenumber(5000.0), **kw)
+    xb = torch.cat([x, x], 0).contiguous()
+    want = torch.cat([fresh.matvec(x, prec=F.FMB_MIXED), p2.matvec(x, prec=F.FMB_MIXED)], 0).cpu().numpy()
+    got = F.matvec_batch([fresh, p2], [2, 2], xb, graph=True, prec=F.FMB_MIXED, reps=3).cpu().numpy()
+    for rep, y in enumerate(got):
+        err = np.linalg.norm(y - want) / np.linalg.norm(want)
+        print("graph replay", rep, "rel err %.2e" % err, flush=True)
+        assert err < 1e-6
