@@ -15,7 +15,10 @@
  * rotations on the host).  Every restart recomputes the true residual
  * b - A x with the FP64 FMM pass; on an FMB_MIXED plan the Arnoldi matvecs use
  * the FP32 pass, so each restart cycle is one step of iterative refinement
- * on the FP64 residual. */
+ * on the FP64 residual.  Their Arnoldi cycles stop when the estimated
+ * residual reaches half the tolerance: the FP32 pass leaves the true residual
+ * a few per cent above the estimate, so a stop right at the tolerance could
+ * fail the FP64 check and pay a second FP64 matvec and a restart. */
 #include <complex.h>
 #include <math.h>
 #include <stdlib.h>
@@ -212,6 +215,7 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
   double t0 = now_s();
   const int N = pl->tree->nelem, m = restart;
   const int work = pl->work_prec;
+  const double itol = work == FMB_MIXED ? 0.5 * tol : tol; /* Arnoldi stop */
   size_t vbytes = sizeof(fmb_c64) * (size_t)N * nrhs;
   fmb_c64 *dV = NULL, *dw = NULL, *dr = NULL, *dh = NULL, *dh2 = NULL, *dy = NULL;
   double *dn = NULL;
@@ -360,7 +364,7 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
         zc gj = g[(size_t)r * (m + 1) + j];
         g[(size_t)r * (m + 1) + j + 1] = -sv * gj;
         g[(size_t)r * (m + 1) + j] = conj(c) * gj;
-        if (active[r] && cabs(g[(size_t)r * (m + 1) + j + 1]) / bn[r] > tol)
+        if (active[r] && cabs(g[(size_t)r * (m + 1) + j + 1]) / bn[r] > itol)
           all_done = 0;
       }
       if ((rc = fmbh_h2d(dh, hh, sizeof(fmb_c64) * nrhs, stream))) goto done;
@@ -376,7 +380,7 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
         for (int r = S.off[p]; r < S.off[p + 1]; r++)
           if (active[r]) {
             any = 1;
-            if (cabs(g[(size_t)r * (m + 1) + j + 1]) / bn[r] > tol) ok = 0;
+            if (cabs(g[(size_t)r * (m + 1) + j + 1]) / bn[r] > itol) ok = 0;
           }
         if (!S.live[p] || !any || !ok) continue;
         for (int r = 0; r < nrhs; r++)
