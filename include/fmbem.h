@@ -251,7 +251,9 @@ typedef struct fmb_gmres_info {
 /* GMRES(restart) to ||b - A x|| <= tol ||b|| (FP64 true residual), FP64
  * Arnoldi with classical Gram-Schmidt and reorthogonalisation.  On a
  * FMB_MIXED plan: iterative refinement with FP64 residuals around GMRES
- * on the FP32 pass.  x holds the initial guess.  The reference has none. */
+ * on the FP32 pass, whose cycles stop at an estimated residual of tol / 2
+ * (the FP64 check still applies tol).  x holds the initial guess.  The
+ * reference has none. */
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream);
