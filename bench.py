@@ -5,7 +5,9 @@ A step is one FMM-BEM matvec (Burton-Miller operator: near field + P2M, M2M,
 M2L, L2L, L2P) over all right-hand sides of the workload; value = right-hand-
 side matvecs per second of the mixed (FP32 pass) path with the inputs already
 in HBM.  The FP64 path is timed the same way, then both solve to a 1e-6 true
-(FP64) residual with GMRES(30).
+(FP64) residual with GMRES(30).  A third solve reaches the same FP64 residual
+of the same operator by iterative refinement around GMRES on an inner plan of
+the same tree with a looser truncation (fmb_gmres_refined, --inner-eps).
 
 Workloads (synthetic geometry generated in-process):
   sphere-l7-10k : icosphere level 7 (327,680 triangles), r = 0.5 m, 10 kHz,
@@ -141,6 +143,10 @@ def main():
                     help="fixed tile-shape rule instead of timing the shapes at plan creation")
     ap.add_argument("--split-bf16", action="store_true",
                     help="split-BF16 rotations in the FP32 pass's LDS-mode translations")
+    ap.add_argument("--inner-eps", type=float, default=0.5,
+                    help="truncation eps of the refined solve's inner plan (0: no refined solve)")
+    ap.add_argument("--inner-tol", type=float, default=1e-3,
+                    help="relative tolerance of each inner solve of the refined solve")
     ap.add_argument("--serial-levels", action="store_true",
                     help="all translations on one stream (per-level times without overlap)")
     a = ap.parse_args()
@@ -214,6 +220,29 @@ def main():
                                                         info.iters, solve[name]["relres"]))
         u64, u32 = solve["fp64"]["x"].cpu().numpy(), solve["mixed"]["x"].cpu().numpy()
         err_mixed = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+        err_ref = None
+        if a.inner_eps > 0:
+            t0 = time.perf_counter()
+            inner = F.Plan(tree, k, precision=F.FMB_MIXED, nrhs_max=nrhs, eps=a.inner_eps,
+                           autotune=0 if a.no_autotune else 1,
+                           serial_levels=1 if a.serial_levels else 0)
+            inner.set_work_precision(F.FMB_MIXED)
+            torch.cuda.synchronize()
+            isetup = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            xs, info = plan.gmres_refined(inner, b, restart=30, tol=1e-6, inner_tol=a.inner_tol,
+                                          maxit=3000, check=False)
+            torch.cuda.synchronize()
+            solve["refined"] = dict(
+                seconds=time.perf_counter() - t0, inner_eps=a.inner_eps, inner_tol=a.inner_tol,
+                inner_orders=[int(q) for q in inner.orders()[2:w["depth"] + 1]],
+                inner_setup_s=isetup, cycles=info.restarts, inner_iters=info.iters,
+                inner_matvecs=info.matvecs_work, fp64_matvecs=info.matvecs_fp64,
+                relres=max(info.relres[r] for r in range(nrhs)))
+            log("solve refined %.2fs cycles %d relres %.2e" % (
+                solve["refined"]["seconds"], info.restarts, solve["refined"]["relres"]))
+            err_ref = float(np.linalg.norm(xs.cpu().numpy() - u64) / np.linalg.norm(u64))
+            del inner
         import oracle as O
         tri = V[Fc]
         c = tri.mean(1)
@@ -222,7 +251,7 @@ def main():
         mie = O.mie_rigid(k * w["radius"], ct, nterms=int(k * w["radius"]) + 40)
         err_mie = float(np.linalg.norm(u64[0] - mie) / np.linalg.norm(mie))
         for s in solve.values():
-            s.pop("x")
+            s.pop("x", None)
 
     cpu = None
     if not a.no_cpu:
@@ -287,6 +316,10 @@ def main():
         out["solve"] = {kk: {q: v for q, v in s.items()} for kk, s in solve.items()}
         out["solve"]["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
         out["solve"]["surface_pressure_rel_err_mixed_vs_fp64"] = err_mixed
+        if "refined" in solve:
+            out["solve"]["speedup_refined_vs_fp64"] = (solve["fp64"]["seconds"] /
+                                                       solve["refined"]["seconds"])
+            out["solve"]["surface_pressure_rel_err_refined_vs_fp64"] = err_ref
         out["solve"]["surface_pressure_rel_err_fp64_vs_mie"] = err_mie
     print(json.dumps(out))
 

@@ -126,6 +126,9 @@ _SIGS = {
     "fmb_sweep_guess": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "fmb_gmres": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.POINTER(GmresInfo), _vp]),
+    "fmb_gmres_refined": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(GmresInfo), _vp]),
     "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fmb_plan_destroy": (None, [_vp]),
     "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -426,6 +429,22 @@ class Plan:
         _check(lib().fmb_matvec_transpose(self.h, prec, parts, _dev(x), _dev(y), nrhs,
                                           _stream()))
         return y
+
+    def gmres_refined(self, inner, b, x0=None, restart=30, tol=1e-6, inner_tol=1e-3,
+                      maxit=1000, check=True):
+        """Solves A x = b to tol on this plan's FP64 pass by iterative
+        refinement around GMRES on `inner`, a plan of the same tree,
+        wavenumber and boundary condition, typically with a looser
+        truncation (fmb_gmres_refined)."""
+        import torch
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        info = GmresInfo()
+        rc = lib().fmb_gmres_refined(self.h, inner.h, _dev(b), _dev(x), nrhs, restart, tol,
+                                     inner_tol, maxit, ctypes.byref(info), _stream())
+        if check:
+            _check(rc)
+        return x, info
 
     def gmres_t(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
         """Solves A^T x = b (fmb_gmres_transpose)."""

@@ -483,6 +483,141 @@ int fmb_gmres_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int gra
   return gmres_run(plans, nrhs, nplans, 0, graph, b, x, restart, tol, maxit, info, stream);
 }
 
+/* Robin coefficients of two plans on one tree agree (both rigid or equal) */
+static int same_robin(const fmb_plan *a, const fmb_plan *b, int N, void *stream, int *same) {
+  *same = 0;
+  if (!a->d_robin || !b->d_robin) {
+    *same = !a->d_robin && !b->d_robin;
+    return FMB_OK;
+  }
+  fmb_c64 *ha = (fmb_c64 *)malloc(sizeof(fmb_c64) * 2 * (size_t)N);
+  if (!ha) return fmb_set_error(FMB_ERR_NOMEM, "Robin coefficient copies");
+  int rc = fmbh_d2h(ha, a->d_robin, sizeof(fmb_c64) * (size_t)N, stream);
+  if (!rc) rc = fmbh_d2h(ha + N, b->d_robin, sizeof(fmb_c64) * (size_t)N, stream);
+  if (!rc) *same = !memcmp(ha, ha + N, sizeof(fmb_c64) * (size_t)N);
+  free(ha);
+  return rc;
+}
+
+/* Iterative refinement on the outer plan's FP64 residual around GMRES on an
+ * inner plan of the same tree, wavenumber, coupling and Robin coefficients
+ * (typically built with a looser truncation, so its passes are cheaper).
+ * Each cycle solves A_in d = r to inner_tol relative to |r| (fmb_gmres on
+ * the inner plan, its own work precision) and sets x += d; right-hand sides
+ * whose outer residual meets tol sit out with r = 0.  The contraction per
+ * cycle is about inner_tol plus the inner operator's relative difference
+ * from the outer one. */
+int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
+                      int nrhs, int restart, double tol, double inner_tol, int maxit,
+                      fmb_gmres_info *info, void *stream) {
+  if (!outer || !inner || !b || !x || nrhs < 1 || restart < 1 || !(tol > 0) ||
+      !(inner_tol > 0 && inner_tol < 1) || maxit < 1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: invalid arguments");
+  if (nrhs > outer->nrhs_max || nrhs > inner->nrhs_max)
+    return fmb_set_error(FMB_ERR_RANGE, "fmb_gmres_refined: %d right-hand sides above a plan's nrhs_max",
+                         nrhs);
+  if (inner->tree != outer->tree || fabs(inner->k - outer->k) > 1e-12 * fabs(outer->k) ||
+      inner->alpha_re != outer->alpha_re || inner->alpha_im != outer->alpha_im)
+    return fmb_set_error(FMB_ERR_ARG,
+                         "fmb_gmres_refined: the inner plan must share the outer plan's tree, "
+                         "wavenumber and coupling");
+  const int N = outer->tree->nelem;
+  int same = 0, rc = same_robin(outer, inner, N, stream, &same);
+  if (rc) return rc;
+  if (!same)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: the plans' Robin coefficients differ");
+  double t0 = now_s();
+  const size_t vbytes = sizeof(fmb_c64) * (size_t)N * nrhs;
+  fmb_c64 *dw = NULL, *dr = NULL, *dd = NULL, *dh = NULL, *dh2 = NULL;
+  double *dn = NULL, *bn = NULL, *rn = NULL;
+  fmb_c64 *one = NULL, *mone = NULL;
+  int iters = 0, mv64 = 0, mvin = 0, cycles = 0, conv = 0, x_zero = 1;
+  if ((rc = fmbh_malloc((void **)&dw, vbytes))) goto done;
+  if ((rc = fmbh_malloc((void **)&dr, vbytes))) goto done;
+  if ((rc = fmbh_malloc((void **)&dd, vbytes))) goto done;
+  if ((rc = fmbh_malloc((void **)&dh, sizeof(fmb_c64) * nrhs))) goto done;
+  if ((rc = fmbh_malloc((void **)&dh2, sizeof(fmb_c64) * nrhs))) goto done;
+  if ((rc = fmbh_malloc((void **)&dn, sizeof(double) * nrhs))) goto done;
+  bn = (double *)malloc(sizeof(double) * nrhs);
+  rn = (double *)malloc(sizeof(double) * nrhs);
+  one = (fmb_c64 *)malloc(sizeof(fmb_c64) * nrhs);
+  mone = (fmb_c64 *)malloc(sizeof(fmb_c64) * nrhs);
+  if (!bn || !rn || !one || !mone) {
+    rc = fmb_set_error(FMB_ERR_NOMEM, "fmb_gmres_refined host workspace");
+    goto done;
+  }
+  for (int r = 0; r < nrhs; r++) {
+    one[r].re = 1.0;
+    one[r].im = 0.0;
+    mone[r].re = -1.0;
+    mone[r].im = 0.0;
+  }
+  if ((rc = fmbh_h2d(dh, one, sizeof(fmb_c64) * nrhs, stream))) goto done;
+  if ((rc = fmbh_h2d(dh2, mone, sizeof(fmb_c64) * nrhs, stream))) goto done;
+  if ((rc = fmbh_znorms(x, nrhs, N, dn, stream))) goto done;
+  if ((rc = fmbh_d2h(rn, dn, sizeof(double) * nrhs, stream))) goto done;
+  for (int r = 0; r < nrhs; r++) x_zero &= rn[r] == 0.0;
+  if ((rc = fmbh_znorms(b, nrhs, N, dn, stream))) goto done;
+  if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
+  for (int r = 0; r < nrhs; r++)
+    if (bn[r] == 0.0) bn[r] = 1.0;
+  while (1) {
+    /* r = b - A_out x in FP64 (r = b for a zero first guess) */
+    if ((rc = fmbh_d2d(dr, b, vbytes, stream))) goto done;
+    if (!x_zero) {
+      if ((rc = fmb_matvec(outer, FMB_FP64, x, dw, nrhs, stream))) goto done;
+      mv64++;
+      if ((rc = fmbh_zaxpby(dh2, dw, dh, dr, nrhs, N, stream))) goto done;
+    }
+    if ((rc = fmbh_znorms(dr, nrhs, N, dn, stream))) goto done;
+    if ((rc = fmbh_d2h(rn, dn, sizeof(double) * nrhs, stream))) goto done;
+    conv = 1;
+    for (int r = 0; r < nrhs; r++) conv &= rn[r] / bn[r] <= tol;
+    if (conv || iters >= maxit) break;
+    for (int r = 0; r < nrhs; r++)
+      if (rn[r] / bn[r] <= tol &&
+          (rc = fmbh_memset(dr + (size_t)r * N, 0, sizeof(fmb_c64) * (size_t)N, stream)))
+        goto done;
+    if ((rc = fmbh_memset(dd, 0, vbytes, stream))) goto done;
+    fmb_gmres_info ii;
+    memset(&ii, 0, sizeof(ii));
+    rc = gmres_run(&inner, &nrhs, 1, 0, 0, dr, dd, restart, inner_tol, maxit - iters, &ii,
+                   stream);
+    if (rc && rc != FMB_ERR_NOCONV) goto done;
+    iters += ii.iters;
+    mvin += ii.matvecs_fp64 + ii.matvecs_work;
+    cycles++;
+    /* x = 1 * d + 1 * x */
+    if ((rc = fmbh_zaxpby(dh, dd, dh, x, nrhs, N, stream))) goto done;
+    x_zero = 0;
+  }
+  rc = conv ? FMB_OK
+            : fmb_set_error(FMB_ERR_NOCONV,
+                            "fmb_gmres_refined: tolerance not reached in %d inner steps", iters);
+  if (info) {
+    memset(info, 0, sizeof(*info));
+    info->iters = iters;
+    info->matvecs_fp64 = mv64;
+    info->matvecs_work = mvin;
+    info->restarts = cycles;
+    for (int r = 0; r < nrhs && r < 64; r++) info->relres[r] = rn[r] / bn[r];
+  }
+done:
+  if (info) info->seconds = now_s() - t0;
+  fmbh_sync(stream);
+  fmbh_free(dw);
+  fmbh_free(dr);
+  fmbh_free(dd);
+  fmbh_free(dh);
+  fmbh_free(dh2);
+  fmbh_free(dn);
+  free(bn);
+  free(rn);
+  free(one);
+  free(mone);
+  return rc;
+}
+
 int fmb_matvec_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
                      int prec, const fmb_c64 *x, fmb_c64 *y, int reps, void *stream) {
   if (!x || !y || reps < 1)

@@ -257,6 +257,20 @@ typedef struct fmb_gmres_info {
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream);
+/* ||b - A_outer x|| <= tol ||b|| (outer plan's FP64 pass) by iterative
+ * refinement around fmb_gmres on an inner plan: each cycle solves
+ * A_inner d = r to inner_tol (0 < inner_tol < 1, relative to |r|) and sets
+ * x += d.  The inner plan must share the outer plan's tree, wavenumber,
+ * coupling and Robin coefficients (FMB_ERR_ARG otherwise); built with a
+ * looser truncation (fmb_plan_params.eps) its passes are cheaper, and each
+ * cycle contracts the residual by about inner_tol plus the relative
+ * difference of the two operators.  maxit bounds the inner Arnoldi steps.
+ * info: iters = inner Arnoldi steps, matvecs_fp64 = outer FP64 passes,
+ * matvecs_work = inner plan passes (either precision), restarts = cycles,
+ * relres = final outer true residuals.  The reference has no solver. */
+int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
+                      int nrhs, int restart, double tol, double inner_tol, int maxit,
+                      fmb_gmres_info *info, void *stream);
 /* Initial guess for a sweep step from the solution x_from at wavenumber
  * k_from: x0_j = x_from_j e^{i (k - k_from) d_r . c_j} with c_j the centroid
  * of element j and d_r the unit plane-wave direction of right-hand side r

@@ -1,0 +1,98 @@
+"""Refinement of the FP64 operator around GMRES on a lower-order inner plan
+(fmb_gmres_refined) on the GPU.
+
+The 5,120-triangle sphere (radius 0.1 m, depth 5) at 8 kHz, three plane
+waves, rigid and with a Robin admittance.  The outer plan has the default
+truncation (eps 0.05); the inner plan is built on the same tree with eps 0.2
+or 0.5, so its leaf order is lower.  Checks:
+* the residual b - A x recomputed here with the outer plan's FP64 pass meets
+  the tolerance and equals the reported one;
+* the solution matches an FP64 GMRES solve of the outer plan (1e-7);
+* one outer FP64 pass per cycle (a zero first guess needs none), and a
+  converged first guess returns after a single check;
+* plans that differ in tree, wavenumber or Robin coefficients are refused
+  before any pass runs."""
+import numpy as np
+import pytest
+import torch
+
+import fmbem_gpu_c_amd as F
+from conftest import wavenumber
+from test_oracle_adjoint import admittance
+
+pytestmark = pytest.mark.gpu
+
+FREQ = 8000.0
+
+
+def _dirs(n):
+    i = np.arange(n) + 0.5
+    z = 1 - 2 * i / n
+    r = np.sqrt(1 - z * z)
+    ph = np.pi * (1 + 5 ** 0.5) * i
+    return np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    m = F.Mesh.icosphere(4, 0.1)
+    return m, F.Tree(m, depth=5)
+
+
+@pytest.mark.parametrize("eps_in,robin", [(0.2, False), (0.5, False), (0.2, True)])
+def test_refined_solve(sphere, eps_in, robin):
+    m, t = sphere
+    k = wavenumber(FREQ)
+    n = m.arrays()[1].shape[0]
+    sig = torch.tensor(1j * k * admittance(n, 2), device="cuda") if robin else None
+    outer = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED, nrhs_max=3)
+    inner = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED, nrhs_max=3, eps=eps_in)
+    po, pi = outer.orders(), inner.orders()
+    print("orders outer", po[2:6].tolist(), "inner", pi[2:6].tolist(), flush=True)
+    assert pi[5] < po[5]
+    b = outer.rhs_plane_wave(_dirs(3))
+    tol = 1e-9
+    x, info = outer.gmres_refined(inner, b, tol=tol, inner_tol=1e-3)
+    print("cycles", info.restarts, "inner steps", info.iters, "outer fp64", info.matvecs_fp64,
+          "inner passes", info.matvecs_work, flush=True)
+    r = b - outer.matvec(x, prec=F.FMB_FP64)
+    rel = (torch.linalg.vector_norm(r, dim=1) / torch.linalg.vector_norm(b, dim=1)).cpu().numpy()
+    print("true relres", rel, flush=True)
+    assert (rel <= tol).all(), rel
+    np.testing.assert_allclose(rel, [info.relres[i] for i in range(3)], rtol=1e-4, atol=0)
+    assert info.restarts >= 2 and info.matvecs_fp64 == info.restarts
+    outer.set_work_precision(F.FMB_FP64)
+    xr, _ = outer.gmres(b, tol=1e-11)
+    err = (torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr)).item()
+    print("vs FP64 GMRES", err, flush=True)
+    assert err < 1e-7, err
+    # a converged first guess: one outer check, no cycle
+    x2, info2 = outer.gmres_refined(inner, b, x0=x, tol=tol, inner_tol=1e-3)
+    assert info2.restarts == 0 and info2.matvecs_fp64 == 1 and info2.iters == 0
+    assert torch.equal(x2, x)
+
+
+def test_refined_rejects_mismatched_plans(sphere):
+    m, t = sphere
+    k = wavenumber(FREQ)
+    n = m.arrays()[1].shape[0]
+    outer = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=1)
+    b = outer.rhs_plane_wave(_dirs(1))
+    sig = torch.tensor(1j * k * admittance(n, 2), device="cuda")
+    others = {
+        "tree": F.Plan(F.Tree(m, depth=4), k, nrhs_max=1, eps=0.2),
+        "wavenumber": F.Plan(t, 1.01 * k, nrhs_max=1, eps=0.2),
+        "robin": F.Plan(t, k, robin=sig, nrhs_max=1, eps=0.2),
+    }
+    for name, inner in others.items():
+        with pytest.raises(F.FmbError, match="inner plan|Robin"):
+            outer.gmres_refined(inner, b)
+        print("refused:", name, flush=True)
+    robin_outer = F.Plan(t, k, robin=sig, nrhs_max=1)
+    robin_other = F.Plan(t, k, robin=2 * sig, nrhs_max=1, eps=0.2)
+    with pytest.raises(F.FmbError, match="Robin"):
+        robin_outer.gmres_refined(robin_other, b)
+    inner = F.Plan(t, k, nrhs_max=1, eps=0.2)
+    for it in (0.0, 1.0, -1e-3):
+        with pytest.raises(F.FmbError, match="invalid"):
+            outer.gmres_refined(inner, b, inner_tol=it)
