@@ -199,9 +199,12 @@ static int update_x(const zc *Hh, const zc *g, zc *yv, fmb_c64 *hh, int m, int n
   return rc;
 }
 
+/* est_stop: a cycle whose estimated residuals all meet the tolerance ends
+ * the solve without the FP64 true residual (for a caller that checks the
+ * result itself); relres then holds those estimates */
 static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, int graph,
                      const fmb_c64 *b, fmb_c64 *x, int restart, double tol, int maxit,
-                     fmb_gmres_info *info, void *stream) {
+                     int est_stop, fmb_gmres_info *info, void *stream) {
   if (!b || !x || restart < 1 || !(tol > 0) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: invalid arguments");
   sys_t S;
@@ -320,7 +323,7 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
     }
     if ((rc = fmbh_h2d(dh, hh, sizeof(fmb_c64) * nrhs, stream))) goto done;
     if ((rc = fmbh_zaxpby(dh, dr, NULL, dV, nrhs, N, stream))) goto done;
-    int j;
+    int j, cyc_done = 0;
     for (j = 0; j < m && iters < maxit; j++) {
       fmb_c64 *Vj = dV + (size_t)j * N * nrhs;
       if ((rc = apply(&S, work, Vj, dw, stream))) goto done;
@@ -416,11 +419,18 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
       }
       if (all_done) {
         j++;
+        cyc_done = 1;
         break;
       }
     }
     /* x += V y, y from the triangular solve (inactive rhs get y = 0) */
     if ((rc = update_x(Hh, g, yv, hh, m, nrhs, j, active, dV, dy, x, N, stream))) goto done;
+    if (est_stop && cyc_done) {
+      for (int r = 0; r < nrhs; r++)
+        if (active[r]) rn[r] = cabs(g[(size_t)r * (m + 1) + j]);
+      conv = 1;
+      break;
+    }
   }
   rc = conv ? FMB_OK : fmb_set_error(FMB_ERR_NOCONV,
                                      "GMRES: tolerance not reached in %d steps",
@@ -468,19 +478,19 @@ done:
 int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream) {
-  return gmres_run(&pl, &nrhs, 1, 0, 0, b, x, restart, tol, maxit, info, stream);
+  return gmres_run(&pl, &nrhs, 1, 0, 0, b, x, restart, tol, maxit, 0, info, stream);
 }
 
 int fmb_gmres_transpose(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
                         int restart, double tol, int maxit, fmb_gmres_info *info,
                         void *stream) {
-  return gmres_run(&pl, &nrhs, 1, 1, 0, b, x, restart, tol, maxit, info, stream);
+  return gmres_run(&pl, &nrhs, 1, 1, 0, b, x, restart, tol, maxit, 0, info, stream);
 }
 
 int fmb_gmres_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,
                     const fmb_c64 *b, fmb_c64 *x, int restart, double tol, int maxit,
                     fmb_gmres_info *info, void *stream) {
-  return gmres_run(plans, nrhs, nplans, 0, graph, b, x, restart, tol, maxit, info, stream);
+  return gmres_run(plans, nrhs, nplans, 0, graph, b, x, restart, tol, maxit, 0, info, stream);
 }
 
 /* Robin coefficients of two plans on one tree agree (both rigid or equal) */
@@ -502,8 +512,9 @@ static int same_robin(const fmb_plan *a, const fmb_plan *b, int N, void *stream,
 /* Iterative refinement on the outer plan's FP64 residual around GMRES on an
  * inner plan of the same tree, wavenumber, coupling and Robin coefficients
  * (typically built with a looser truncation, so its passes are cheaper).
- * Each cycle solves A_in d = r to inner_tol relative to |r| (fmb_gmres on
- * the inner plan, its own work precision) and sets x += d; right-hand sides
+ * Each cycle solves A_in d = r to inner_tol relative to |r| (GMRES on the
+ * inner plan in its own work precision, stopped on its estimated residual:
+ * the outer FP64 residual checks every cycle) and sets x += d; right-hand sides
  * whose outer residual meets tol sit out with r = 0.  The contraction per
  * cycle is about inner_tol plus the inner operator's relative difference
  * from the outer one. */
@@ -581,7 +592,7 @@ int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c6
     if ((rc = fmbh_memset(dd, 0, vbytes, stream))) goto done;
     fmb_gmres_info ii;
     memset(&ii, 0, sizeof(ii));
-    rc = gmres_run(&inner, &nrhs, 1, 0, 0, dr, dd, restart, inner_tol, maxit - iters, &ii,
+    rc = gmres_run(&inner, &nrhs, 1, 0, 0, dr, dd, restart, inner_tol, maxit - iters, 1, &ii,
                    stream);
     if (rc && rc != FMB_ERR_NOCONV) goto done;
     iters += ii.iters;
