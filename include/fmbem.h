@@ -258,8 +258,9 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
               int restart, double tol, int maxit, fmb_gmres_info *info,
               void *stream);
 /* ||b - A_outer x|| <= tol ||b|| (outer plan's FP64 pass) by iterative
- * refinement around fmb_gmres on an inner plan: each cycle solves
- * A_inner d = r to inner_tol (0 < inner_tol < 1, relative to |r|) and sets
+ * refinement around GMRES on an inner plan: each cycle solves
+ * A_inner d = r to inner_tol (0 < inner_tol < 1, relative to |r|; the
+ * inner plan's work precision, stopped on the Arnoldi estimate) and sets
  * x += d.  The inner plan must share the outer plan's tree, wavenumber,
  * coupling and Robin coefficients (FMB_ERR_ARG otherwise); built with a
  * looser truncation (fmb_plan_params.eps) its passes are cheaper, and each
