@@ -129,6 +129,10 @@ _SIGS = {
     "fmb_gmres_refined": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.POINTER(GmresInfo), _vp]),
+    "fmb_gmres_refined_transpose": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int,
+                                                   ctypes.POINTER(GmresInfo), _vp]),
     "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fmb_plan_destroy": (None, [_vp]),
     "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -436,12 +440,22 @@ class Plan:
         refinement around GMRES on `inner`, a plan of the same tree,
         wavenumber and boundary condition, typically with a looser
         truncation (fmb_gmres_refined)."""
+        return self._refined(lib().fmb_gmres_refined, inner, b, x0, restart, tol, inner_tol,
+                             maxit, check)
+
+    def gmres_refined_t(self, inner, b, x0=None, restart=30, tol=1e-6, inner_tol=1e-3,
+                        maxit=1000, check=True):
+        """gmres_refined on A^T x = b (fmb_gmres_refined_transpose)."""
+        return self._refined(lib().fmb_gmres_refined_transpose, inner, b, x0, restart, tol,
+                             inner_tol, maxit, check)
+
+    def _refined(self, fn, inner, b, x0, restart, tol, inner_tol, maxit, check):
         import torch
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         nrhs = 1 if b.dim() == 1 else b.shape[0]
         info = GmresInfo()
-        rc = lib().fmb_gmres_refined(self.h, inner.h, _dev(b), _dev(x), nrhs, restart, tol,
-                                     inner_tol, maxit, ctypes.byref(info), _stream())
+        rc = fn(self.h, inner.h, _dev(b), _dev(x), nrhs, restart, tol, inner_tol, maxit,
+                ctypes.byref(info), _stream())
         if check:
             _check(rc)
         return x, info

@@ -518,9 +518,9 @@ static int same_robin(const fmb_plan *a, const fmb_plan *b, int N, void *stream,
  * whose outer residual meets tol sit out with r = 0.  The contraction per
  * cycle is about inner_tol plus the inner operator's relative difference
  * from the outer one. */
-int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
-                      int nrhs, int restart, double tol, double inner_tol, int maxit,
-                      fmb_gmres_info *info, void *stream) {
+static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c64 *b,
+                       fmb_c64 *x, int nrhs, int restart, double tol, double inner_tol,
+                       int maxit, fmb_gmres_info *info, void *stream) {
   if (!outer || !inner || !b || !x || nrhs < 1 || restart < 1 || !(tol > 0) ||
       !(inner_tol > 0 && inner_tol < 1) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: invalid arguments");
@@ -576,7 +576,7 @@ int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c6
     /* r = b - A_out x in FP64 (r = b for a zero first guess) */
     if ((rc = fmbh_d2d(dr, b, vbytes, stream))) goto done;
     if (!x_zero) {
-      if ((rc = fmb_matvec(outer, FMB_FP64, x, dw, nrhs, stream))) goto done;
+      if ((rc = plan_apply(outer, trans, FMB_FP64, x, dw, nrhs, stream))) goto done;
       mv64++;
       if ((rc = fmbh_zaxpby(dh2, dw, dh, dr, nrhs, N, stream))) goto done;
     }
@@ -592,8 +592,8 @@ int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c6
     if ((rc = fmbh_memset(dd, 0, vbytes, stream))) goto done;
     fmb_gmres_info ii;
     memset(&ii, 0, sizeof(ii));
-    rc = gmres_run(&inner, &nrhs, 1, 0, 0, dr, dd, restart, inner_tol, maxit - iters, 1, &ii,
-                   stream);
+    rc = gmres_run(&inner, &nrhs, 1, trans, 0, dr, dd, restart, inner_tol, maxit - iters, 1,
+                   &ii, stream);
     if (rc && rc != FMB_ERR_NOCONV) goto done;
     iters += ii.iters;
     mvin += ii.matvecs_fp64 + ii.matvecs_work;
@@ -627,6 +627,19 @@ done:
   free(one);
   free(mone);
   return rc;
+}
+
+int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
+                      int nrhs, int restart, double tol, double inner_tol, int maxit,
+                      fmb_gmres_info *info, void *stream) {
+  return refined_run(outer, inner, 0, b, x, nrhs, restart, tol, inner_tol, maxit, info, stream);
+}
+
+int fmb_gmres_refined_transpose(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b,
+                                fmb_c64 *x, int nrhs, int restart, double tol,
+                                double inner_tol, int maxit, fmb_gmres_info *info,
+                                void *stream) {
+  return refined_run(outer, inner, 1, b, x, nrhs, restart, tol, inner_tol, maxit, info, stream);
 }
 
 int fmb_matvec_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,

@@ -272,6 +272,12 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
 int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
                       int nrhs, int restart, double tol, double inner_tol, int maxit,
                       fmb_gmres_info *info, void *stream);
+/* fmb_gmres_refined on A^T x = b (transposed passes of both plans; same
+ * options, checks and info).  The reference has no adjoint. */
+int fmb_gmres_refined_transpose(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b,
+                                fmb_c64 *x, int nrhs, int restart, double tol,
+                                double inner_tol, int maxit, fmb_gmres_info *info,
+                                void *stream);
 /* Initial guess for a sweep step from the solution x_from at wavenumber
  * k_from: x0_j = x_from_j e^{i (k - k_from) d_r . c_j} with c_j the centroid
  * of element j and d_r the unit plane-wave direction of right-hand side r

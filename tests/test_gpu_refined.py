@@ -11,7 +11,9 @@ or 0.5, so its leaf order is lower.  Checks:
 * one outer FP64 pass per cycle (a zero first guess needs none), and a
   converged first guess returns after a single check;
 * plans that differ in tree, wavenumber or Robin coefficients are refused
-  before any pass runs."""
+  before any pass runs;
+* the transposed form (fmb_gmres_refined_transpose) against the transposed
+  FP64 pass and fmb_gmres_transpose."""
 import numpy as np
 import pytest
 import torch
@@ -96,3 +98,34 @@ def test_refined_rejects_mismatched_plans(sphere):
     for it in (0.0, 1.0, -1e-3):
         with pytest.raises(F.FmbError, match="invalid"):
             outer.gmres_refined(inner, b, inner_tol=it)
+
+
+def test_refined_transpose(sphere):
+    """A^T x = b with a Robin admittance (A^T differs from A): the residual
+    recomputed with the outer plan's transposed FP64 pass, and the solution
+    against fmb_gmres_transpose with FP64 Arnoldi."""
+    m, t = sphere
+    k = wavenumber(FREQ)
+    n = m.arrays()[1].shape[0]
+    sig = torch.tensor(1j * k * admittance(n, 2), device="cuda")
+    outer = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED, nrhs_max=2)
+    inner = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED, nrhs_max=2, eps=0.5)
+    rng = np.random.default_rng(7)
+    b = torch.tensor(rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)), device="cuda")
+    tol = 1e-9
+    x, info = outer.gmres_refined_t(inner, b, tol=tol, inner_tol=1e-3)
+    print("cycles", info.restarts, "inner steps", info.iters, "outer fp64", info.matvecs_fp64,
+          flush=True)
+    r = b - outer.matvec_t(x, prec=F.FMB_FP64)
+    rel = (torch.linalg.vector_norm(r, dim=1) / torch.linalg.vector_norm(b, dim=1)).cpu().numpy()
+    print("true relres", rel, flush=True)
+    assert (rel <= tol).all(), rel
+    assert info.matvecs_fp64 == info.restarts >= 2
+    outer.set_work_precision(F.FMB_FP64)
+    xr, _ = outer.gmres_t(b, tol=1e-11)
+    err = (torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr)).item()
+    print("vs FP64 transposed GMRES", err, flush=True)
+    assert err < 1e-7, err
+    # the forward solve of the same system is a different solution
+    xf, _ = outer.gmres_refined(inner, b, tol=tol, inner_tol=1e-3)
+    assert (torch.linalg.vector_norm(xf - x) / torch.linalg.vector_norm(x)).item() > 1e-3
