@@ -98,6 +98,11 @@ def test_refined_rejects_mismatched_plans(sphere):
     for it in (0.0, 1.0, -1e-3):
         with pytest.raises(F.FmbError, match="invalid"):
             outer.gmres_refined(inner, b, inner_tol=it)
+    # more right-hand sides than the inner plan holds
+    outer3 = F.Plan(t, k, nrhs_max=3)
+    b3 = outer3.rhs_plane_wave(_dirs(3))
+    with pytest.raises(F.FmbError, match="nrhs_max"):
+        outer3.gmres_refined(inner, b3)
 
 
 def test_refined_transpose(sphere):
