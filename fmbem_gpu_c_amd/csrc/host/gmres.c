@@ -525,8 +525,8 @@ static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c6
       !(inner_tol > 0 && inner_tol < 1) || maxit < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: invalid arguments");
   if (nrhs > outer->nrhs_max || nrhs > inner->nrhs_max)
-    return fmb_set_error(FMB_ERR_RANGE, "fmb_gmres_refined: %d right-hand sides above a plan's nrhs_max",
-                         nrhs);
+    return fmb_set_error(FMB_ERR_RANGE,
+                         "fmb_gmres_refined: %d right-hand sides above a plan's nrhs_max", nrhs);
   if (inner->tree != outer->tree || fabs(inner->k - outer->k) > 1e-12 * fabs(outer->k) ||
       inner->alpha_re != outer->alpha_re || inner->alpha_im != outer->alpha_im)
     return fmb_set_error(FMB_ERR_ARG,
