@@ -133,6 +133,9 @@ _SIGS = {
                                                    ctypes.c_int, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int,
                                                    ctypes.POINTER(GmresInfo), _vp]),
+    "fmb_gmres_batch_refined": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                               _vp, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, _vp, _vp]),
     "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fmb_plan_destroy": (None, [_vp]),
     "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -573,6 +576,27 @@ def gmres_batch(plans, nrhs, b, x0=None, graph=True, restart=30, tol=1e-6, maxit
     infos = (GmresInfo * len(plans))()
     rc = lib().fmb_gmres_batch(arr, nr, len(plans), int(graph), _dev(b), _dev(x), restart,
                                tol, maxit, infos, _stream())
+    if check:
+        _check(rc)
+    return x, list(infos)
+
+
+def gmres_batch_refined(outers, inners, nrhs, b, x0=None, graph=True, restart=30, tol=1e-6,
+                        inner_tol=1e-3, maxit=1000, check=True):
+    """fmb_gmres_batch_refined: refinement on each outer plan's FP64 residual
+    around GMRES on the inner plans in lock step (inner p on outer p's tree,
+    wavenumber and boundary condition).  b and the result as gmres_batch."""
+    import torch
+    nrhs = [int(r) for r in nrhs]
+    no = (_vp * len(outers))(*[p.h for p in outers])
+    ni = (_vp * len(inners))(*[p.h for p in inners])
+    if len(inners) != len(outers):
+        raise FmbError(-1, "one inner plan per outer plan")
+    nr = (ctypes.c_int * len(outers))(*nrhs)
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    infos = (GmresInfo * len(outers))()
+    rc = lib().fmb_gmres_batch_refined(no, ni, nr, len(outers), int(graph), _dev(b), _dev(x),
+                                       restart, tol, inner_tol, maxit, infos, _stream())
     if check:
         _check(rc)
     return x, list(infos)

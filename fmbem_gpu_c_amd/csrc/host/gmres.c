@@ -393,6 +393,16 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
         for (int r = S.off[p]; r < S.off[p + 1]; r++) active[r] = 0;
         const size_t o = (size_t)S.off[p] * N;
         const int nq = nrs[p];
+        if (est_stop) {
+          /* the caller checks the result: retire on the estimates */
+          for (int r = S.off[p]; r < S.off[p + 1]; r++)
+            rsave[r] = sel[r] ? cabs(g[(size_t)r * (m + 1) + j + 1]) : rn[r];
+          S.live[p] = 0;
+          graphs_drop(&S);
+          if ((rc = fmbh_memset(dw + o, 0, sizeof(fmb_c64) * nq * (size_t)N, stream)))
+            goto done;
+          continue;
+        }
         /* w's columns of p are free (V_{j+1} is formed): A x_p there */
         if ((rc = plan_apply(S.pl[p], trans, FMB_FP64, x + o, dw + o, nq, stream))) goto done;
         mv64++;
@@ -428,6 +438,9 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
     if (est_stop && cyc_done) {
       for (int r = 0; r < nrhs; r++)
         if (active[r]) rn[r] = cabs(g[(size_t)r * (m + 1) + j]);
+      for (int p = 0; p < np; p++)
+        if (!S.live[p])
+          for (int r = S.off[p]; r < S.off[p + 1]; r++) rn[r] = rsave[r];
       conv = 1;
       break;
     }
@@ -509,40 +522,59 @@ static int same_robin(const fmb_plan *a, const fmb_plan *b, int N, void *stream,
   return rc;
 }
 
-/* Iterative refinement on the outer plan's FP64 residual around GMRES on an
- * inner plan of the same tree, wavenumber, coupling and Robin coefficients
- * (typically built with a looser truncation, so its passes are cheaper).
- * Each cycle solves A_in d = r to inner_tol relative to |r| (GMRES on the
- * inner plan in its own work precision, stopped on its estimated residual:
- * the outer FP64 residual checks every cycle) and sets x += d; right-hand sides
- * whose outer residual meets tol sit out with r = 0.  The contraction per
- * cycle is about inner_tol plus the inner operator's relative difference
- * from the outer one. */
-static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c64 *b,
-                       fmb_c64 *x, int nrhs, int restart, double tol, double inner_tol,
-                       int maxit, fmb_gmres_info *info, void *stream) {
-  if (!outer || !inner || !b || !x || nrhs < 1 || restart < 1 || !(tol > 0) ||
-      !(inner_tol > 0 && inner_tol < 1) || maxit < 1)
+/* the inner plan of a refined solve agrees with its outer plan */
+static int refined_pair_check(const fmb_plan *outer, const fmb_plan *inner, int nr, void *stream) {
+  if (!outer || !inner || nr < 1)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: invalid arguments");
-  if (nrhs > outer->nrhs_max || nrhs > inner->nrhs_max)
+  if (nr > outer->nrhs_max || nr > inner->nrhs_max)
     return fmb_set_error(FMB_ERR_RANGE,
-                         "fmb_gmres_refined: %d right-hand sides above a plan's nrhs_max", nrhs);
+                         "fmb_gmres_refined: %d right-hand sides above a plan's nrhs_max", nr);
   if (inner->tree != outer->tree || fabs(inner->k - outer->k) > 1e-12 * fabs(outer->k) ||
       inner->alpha_re != outer->alpha_re || inner->alpha_im != outer->alpha_im)
     return fmb_set_error(FMB_ERR_ARG,
                          "fmb_gmres_refined: the inner plan must share the outer plan's tree, "
                          "wavenumber and coupling");
-  const int N = outer->tree->nelem;
-  int same = 0, rc = same_robin(outer, inner, N, stream, &same);
+  int same = 0, rc = same_robin(outer, inner, outer->tree->nelem, stream, &same);
   if (rc) return rc;
   if (!same)
     return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: the plans' Robin coefficients differ");
+  return FMB_OK;
+}
+
+/* Iterative refinement on each outer plan's FP64 residual around GMRES on
+ * its inner plan, a plan of the same tree, wavenumber, coupling and Robin
+ * coefficients (typically built with a looser truncation, so its passes are
+ * cheaper).  Plan p owns columns [off[p], off[p+1]) of b and x.  Each cycle
+ * solves A_in d = r to inner_tol relative to |r| (GMRES on the inner plans in
+ * lock step, in their work precision, stopped on the estimated residual: the
+ * outer FP64 residual checks every cycle) and sets x += d.  Right-hand sides
+ * whose outer residual meets tol sit out with r = 0, and a plan whose
+ * right-hand sides all do needs no further outer pass.  The contraction per
+ * cycle is about inner_tol plus the inner operator's relative difference
+ * from the outer one. */
+static int refined_run(fmb_plan *const *outers, fmb_plan *const *inners, const int *nrs, int np,
+                       int trans, int graph, const fmb_c64 *b, fmb_c64 *x, int restart,
+                       double tol, double inner_tol, int maxit, fmb_gmres_info *info,
+                       void *stream) {
+  if (!outers || !inners || !nrs || np < 1 || !b || !x || restart < 1 || !(tol > 0) ||
+      !(inner_tol > 0 && inner_tol < 1) || maxit < 1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: invalid arguments");
+  int rc, nrhs = 0;
+  for (int p = 0; p < np; p++) {
+    if ((rc = refined_pair_check(outers[p], inners[p], nrs[p], stream))) return rc;
+    if (outers[p]->tree->nelem != outers[0]->tree->nelem)
+      return fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: plans of different element counts");
+    nrhs += nrs[p];
+  }
+  const int N = outers[0]->tree->nelem;
   double t0 = now_s();
   const size_t vbytes = sizeof(fmb_c64) * (size_t)N * nrhs;
   fmb_c64 *dw = NULL, *dr = NULL, *dd = NULL, *dh = NULL, *dh2 = NULL;
-  double *dn = NULL, *bn = NULL, *rn = NULL;
+  double *dn = NULL, *bn = NULL, *rn = NULL, *tn = NULL;
   fmb_c64 *one = NULL, *mone = NULL;
-  int iters = 0, mv64 = 0, mvin = 0, cycles = 0, conv = 0, x_zero = 1;
+  fmb_gmres_info *ii = NULL;
+  int *off = NULL, *pdone = NULL, *mvin = NULL, *mv64 = NULL;
+  int iters = 0, cycles = 0, conv = 0, x_zero = 1;
   if ((rc = fmbh_malloc((void **)&dw, vbytes))) goto done;
   if ((rc = fmbh_malloc((void **)&dr, vbytes))) goto done;
   if ((rc = fmbh_malloc((void **)&dd, vbytes))) goto done;
@@ -551,12 +583,20 @@ static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c6
   if ((rc = fmbh_malloc((void **)&dn, sizeof(double) * nrhs))) goto done;
   bn = (double *)malloc(sizeof(double) * nrhs);
   rn = (double *)malloc(sizeof(double) * nrhs);
+  tn = (double *)malloc(sizeof(double) * nrhs);
   one = (fmb_c64 *)malloc(sizeof(fmb_c64) * nrhs);
   mone = (fmb_c64 *)malloc(sizeof(fmb_c64) * nrhs);
-  if (!bn || !rn || !one || !mone) {
+  ii = (fmb_gmres_info *)calloc((size_t)np, sizeof(fmb_gmres_info));
+  off = (int *)malloc(sizeof(int) * (np + 1));
+  pdone = (int *)calloc((size_t)np, sizeof(int));
+  mvin = (int *)calloc((size_t)np, sizeof(int));
+  mv64 = (int *)calloc((size_t)np, sizeof(int));
+  if (!bn || !rn || !tn || !one || !mone || !ii || !off || !pdone || !mvin || !mv64) {
     rc = fmb_set_error(FMB_ERR_NOMEM, "fmb_gmres_refined host workspace");
     goto done;
   }
+  off[0] = 0;
+  for (int p = 0; p < np; p++) off[p + 1] = off[p] + nrs[p];
   for (int r = 0; r < nrhs; r++) {
     one[r].re = 1.0;
     one[r].im = 0.0;
@@ -573,30 +613,45 @@ static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c6
   for (int r = 0; r < nrhs; r++)
     if (bn[r] == 0.0) bn[r] = 1.0;
   while (1) {
-    /* r = b - A_out x in FP64 (r = b for a zero first guess) */
+    /* r = b - A_out x in FP64 (r = b for a zero first guess); converged
+     * plans keep r = 0 */
     if ((rc = fmbh_d2d(dr, b, vbytes, stream))) goto done;
     if (!x_zero) {
-      if ((rc = plan_apply(outer, trans, FMB_FP64, x, dw, nrhs, stream))) goto done;
-      mv64++;
-      if ((rc = fmbh_zaxpby(dh2, dw, dh, dr, nrhs, N, stream))) goto done;
+      for (int p = 0; p < np; p++) {
+        if (pdone[p]) continue;
+        const size_t o = (size_t)off[p] * N;
+        if ((rc = plan_apply(outers[p], trans, FMB_FP64, x + o, dw + o, nrs[p], stream)))
+          goto done;
+        mv64[p]++;
+        if ((rc = fmbh_zaxpby(dh2 + off[p], dw + o, dh + off[p], dr + o, nrs[p], N, stream)))
+          goto done;
+      }
     }
     if ((rc = fmbh_znorms(dr, nrhs, N, dn, stream))) goto done;
-    if ((rc = fmbh_d2h(rn, dn, sizeof(double) * nrhs, stream))) goto done;
+    if ((rc = fmbh_d2h(tn, dn, sizeof(double) * nrhs, stream))) goto done;
+    /* finished plans keep the norms they converged with */
+    for (int p = 0; p < np; p++)
+      if (!pdone[p])
+        for (int r = off[p]; r < off[p + 1]; r++) rn[r] = tn[r];
     conv = 1;
-    for (int r = 0; r < nrhs; r++) conv &= rn[r] / bn[r] <= tol;
+    for (int p = 0; p < np; p++) {
+      int ok = 1;
+      for (int r = off[p]; r < off[p + 1]; r++) ok &= rn[r] / bn[r] <= tol;
+      pdone[p] |= ok;
+      conv &= ok;
+    }
     if (conv || iters >= maxit) break;
     for (int r = 0; r < nrhs; r++)
       if (rn[r] / bn[r] <= tol &&
           (rc = fmbh_memset(dr + (size_t)r * N, 0, sizeof(fmb_c64) * (size_t)N, stream)))
         goto done;
     if ((rc = fmbh_memset(dd, 0, vbytes, stream))) goto done;
-    fmb_gmres_info ii;
-    memset(&ii, 0, sizeof(ii));
-    rc = gmres_run(&inner, &nrhs, 1, trans, 0, dr, dd, restart, inner_tol, maxit - iters, 1,
-                   &ii, stream);
+    memset(ii, 0, sizeof(fmb_gmres_info) * np);
+    rc = gmres_run(inners, nrs, np, trans, graph, dr, dd, restart, inner_tol, maxit - iters, 1,
+                   ii, stream);
     if (rc && rc != FMB_ERR_NOCONV) goto done;
-    iters += ii.iters;
-    mvin += ii.matvecs_fp64 + ii.matvecs_work;
+    iters += ii[0].iters;
+    for (int p = 0; p < np; p++) mvin[p] += ii[p].matvecs_fp64 + ii[p].matvecs_work;
     cycles++;
     /* x = 1 * d + 1 * x */
     if ((rc = fmbh_zaxpby(dh, dd, dh, x, nrhs, N, stream))) goto done;
@@ -605,16 +660,19 @@ static int refined_run(fmb_plan *outer, fmb_plan *inner, int trans, const fmb_c6
   rc = conv ? FMB_OK
             : fmb_set_error(FMB_ERR_NOCONV,
                             "fmb_gmres_refined: tolerance not reached in %d inner steps", iters);
-  if (info) {
-    memset(info, 0, sizeof(*info));
-    info->iters = iters;
-    info->matvecs_fp64 = mv64;
-    info->matvecs_work = mvin;
-    info->restarts = cycles;
-    for (int r = 0; r < nrhs && r < 64; r++) info->relres[r] = rn[r] / bn[r];
-  }
+  if (info)
+    for (int p = 0; p < np; p++) {
+      fmb_gmres_info *ip = info + p;
+      memset(ip, 0, sizeof(*ip));
+      ip->iters = iters;
+      ip->matvecs_fp64 = mv64[p];
+      ip->matvecs_work = mvin[p];
+      ip->restarts = cycles;
+      for (int r = 0; r < nrs[p] && r < 64; r++) ip->relres[r] = rn[off[p] + r] / bn[off[p] + r];
+    }
 done:
-  if (info) info->seconds = now_s() - t0;
+  if (info)
+    for (int p = 0; p < np; p++) info[p].seconds = now_s() - t0;
   fmbh_sync(stream);
   fmbh_free(dw);
   fmbh_free(dr);
@@ -624,22 +682,38 @@ done:
   fmbh_free(dn);
   free(bn);
   free(rn);
+  free(tn);
   free(one);
   free(mone);
+  free(ii);
+  free(off);
+  free(pdone);
+  free(mvin);
+  free(mv64);
   return rc;
 }
 
 int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
                       int nrhs, int restart, double tol, double inner_tol, int maxit,
                       fmb_gmres_info *info, void *stream) {
-  return refined_run(outer, inner, 0, b, x, nrhs, restart, tol, inner_tol, maxit, info, stream);
+  return refined_run(&outer, &inner, &nrhs, 1, 0, 0, b, x, restart, tol, inner_tol, maxit, info,
+                     stream);
 }
 
 int fmb_gmres_refined_transpose(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b,
                                 fmb_c64 *x, int nrhs, int restart, double tol,
                                 double inner_tol, int maxit, fmb_gmres_info *info,
                                 void *stream) {
-  return refined_run(outer, inner, 1, b, x, nrhs, restart, tol, inner_tol, maxit, info, stream);
+  return refined_run(&outer, &inner, &nrhs, 1, 1, 0, b, x, restart, tol, inner_tol, maxit, info,
+                     stream);
+}
+
+int fmb_gmres_batch_refined(fmb_plan *const *outers, fmb_plan *const *inners, const int *nrhs,
+                            int nplans, int graph, const fmb_c64 *b, fmb_c64 *x, int restart,
+                            double tol, double inner_tol, int maxit, fmb_gmres_info *info,
+                            void *stream) {
+  return refined_run(outers, inners, nrhs, nplans, 0, graph, b, x, restart, tol, inner_tol, maxit,
+                     info, stream);
 }
 
 int fmb_matvec_batch(fmb_plan *const *plans, const int *nrhs, int nplans, int graph,

@@ -272,6 +272,16 @@ int fmb_gmres(fmb_plan *pl, const fmb_c64 *b, fmb_c64 *x, int nrhs,
 int fmb_gmres_refined(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b, fmb_c64 *x,
                       int nrhs, int restart, double tol, double inner_tol, int maxit,
                       fmb_gmres_info *info, void *stream);
+/* fmb_gmres_refined over several frequencies in lock step: outers[p] and
+ * inners[p] as the pair of fmb_gmres_refined, columns of b and x as
+ * fmb_gmres_batch (the inner solves run as one fmb_gmres_batch, graph as
+ * there); a plan whose right-hand sides all meet tol takes no further outer
+ * pass.  info: one per plan.  The reference solves one frequency at a
+ * time. */
+int fmb_gmres_batch_refined(fmb_plan *const *outers, fmb_plan *const *inners, const int *nrhs,
+                            int nplans, int graph, const fmb_c64 *b, fmb_c64 *x, int restart,
+                            double tol, double inner_tol, int maxit, fmb_gmres_info *info,
+                            void *stream);
 /* fmb_gmres_refined on A^T x = b (transposed passes of both plans; same
  * options, checks and info).  The reference has no adjoint. */
 int fmb_gmres_refined_transpose(fmb_plan *outer, fmb_plan *inner, const fmb_c64 *b,

@@ -13,7 +13,9 @@ or 0.5, so its leaf order is lower.  Checks:
 * plans that differ in tree, wavenumber or Robin coefficients are refused
   before any pass runs;
 * the transposed form (fmb_gmres_refined_transpose) against the transposed
-  FP64 pass and fmb_gmres_transpose."""
+  FP64 pass and fmb_gmres_transpose;
+* the batch over frequencies (fmb_gmres_batch_refined, graph and eager)
+  against each plan's FP64 residual and individual refined solves."""
 import numpy as np
 import pytest
 import torch
@@ -134,3 +136,41 @@ def test_refined_transpose(sphere):
     # the forward solve of the same system is a different solution
     xf, _ = outer.gmres_refined(inner, b, tol=tol, inner_tol=1e-3)
     assert (torch.linalg.vector_norm(xf - x) / torch.linalg.vector_norm(x)).item() > 1e-3
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_refined_matches_individual(graph):
+    """fmb_gmres_batch_refined over 1, 4 and 9 kHz (tree depths 3, 4, 5 on one
+    ellipsoid mesh, 2, 3 and 1 plane waves, inner plans at eps 0.3): every
+    plan's residual recomputed with its own FP64 pass, and the solutions
+    against individual fmb_gmres_refined solves."""
+    m = F.Mesh.ellipsoid(12, 0.08, 0.07, 0.10, seed=3)
+    n = m.arrays()[1].shape[0]
+    nr = [2, 3, 1]
+    outers, inners, bs, singles = [], [], [], []
+    dirs = _dirs(sum(nr))
+    c = 0
+    for f, dpt, r in zip((1000.0, 4000.0, 9000.0), (3, 4, 5), nr):
+        t = F.Tree(m, depth=dpt)
+        k = wavenumber(f)
+        o = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=r)
+        i = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=r, eps=0.3)
+        b = o.rhs_plane_wave(dirs[c:c + r])
+        c += r
+        x, _ = o.gmres_refined(i, b, tol=1e-9, inner_tol=1e-3)
+        outers.append(o); inners.append(i); bs.append(b); singles.append(x)
+    b = torch.cat(bs, 0).contiguous()
+    x, infos = F.gmres_batch_refined(outers, inners, nr, b, graph=graph, tol=1e-9, inner_tol=1e-3)
+    assert len(infos) == 3
+    c = 0
+    for p, (o, r) in enumerate(zip(outers, nr)):
+        xp = x[c:c + r].contiguous()
+        res = bs[p] - o.matvec(xp, prec=F.FMB_FP64)
+        rel = (torch.linalg.vector_norm(res, dim=1) / torch.linalg.vector_norm(bs[p], dim=1)).cpu().numpy()
+        err = (torch.linalg.vector_norm(xp - singles[p]) / torch.linalg.vector_norm(singles[p])).item()
+        print("plan", p, "cycles", infos[p].restarts, "outer fp64", infos[p].matvecs_fp64,
+              "relres", rel, "vs single %.2e" % err, flush=True)
+        assert (rel <= 1e-9).all(), (p, rel)
+        assert 1 <= infos[p].matvecs_fp64 <= infos[p].restarts
+        assert err < 1e-7, (p, err)
+        c += r
