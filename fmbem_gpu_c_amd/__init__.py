@@ -136,6 +136,10 @@ _SIGS = {
     "fmb_gmres_batch_refined": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
                                                _vp, ctypes.c_int, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    "fmb_adjoint_grad_refined": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_int,
+                                                _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_int, _vp, _vp, _vp,
+                                                _vp, _vp, _vp]),
     "fmb_plan_set_work_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fmb_plan_destroy": (None, [_vp]),
     "fmb_last_timing": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -476,13 +480,14 @@ class Plan:
         return x, info
 
     def adjoint_grad(self, x, pts, dirs=None, restart=30, tol=1e-10, maxit=1000,
-                     vertices=False):
+                     vertices=False, inner=None, inner_tol=1e-3):
         """J = sum |p|^2 at pts (M x 3) for the surface solution x and its
         gradient dJ/dRe sigma + i dJ/dIm sigma per element (complex128 CUDA
         tensor, caller order) with respect to the Robin coefficients
         (fmb_adjoint_grad).  Returns (J, grad, info of the adjoint solve), and
         with vertices=True also dJ/dV (nvert x 3) and dJ/d(vertex normal
-        displacement) (nvert) as float64 CUDA tensors."""
+        displacement) (nvert) as float64 CUDA tensors.  With an inner plan the
+        adjoint solve refines around GMRES on it (fmb_adjoint_grad_refined)."""
         import torch
         nrhs = 1 if x.dim() == 1 else x.shape[0]
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
@@ -500,13 +505,14 @@ class Plan:
             dvn = torch.empty(nv, dtype=torch.float64, device=x.device)
         J = ctypes.c_double()
         info = GmresInfo()
-        _check(lib().fmb_adjoint_grad(self.h, _dev(x), nrhs,
-                                      ctypes.c_void_p(d.ctypes.data) if d is not None else None,
-                                      ctypes.c_void_p(pts.ctypes.data), pts.shape[0], restart,
-                                      tol, maxit, ctypes.byref(J), _dev(g),
-                                      _dev(dV, torch.float64) if vertices else None,
-                                      _dev(dvn, torch.float64) if vertices else None,
-                                      ctypes.byref(info), _stream()))
+        rest = (_dev(x), nrhs, ctypes.c_void_p(d.ctypes.data) if d is not None else None,
+                ctypes.c_void_p(pts.ctypes.data), pts.shape[0], restart, tol, maxit,
+                ctypes.byref(J), _dev(g), _dev(dV, torch.float64) if vertices else None,
+                _dev(dvn, torch.float64) if vertices else None, ctypes.byref(info), _stream())
+        if inner is None:
+            _check(lib().fmb_adjoint_grad(self.h, *rest))
+        else:
+            _check(lib().fmb_adjoint_grad_refined(self.h, inner.h, inner_tol, *rest))
         if vertices:
             return J.value, g, info, dV, dvn
         return J.value, g, info

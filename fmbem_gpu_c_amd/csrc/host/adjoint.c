@@ -28,10 +28,12 @@
 #include "../hip/fmb_hip.h"
 #include "host.h"
 
-int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dirs,
-                     const double *pts, int npts, int restart, double tol, int maxit,
-                     double *J, fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
-                     fmb_gmres_info *info, void *stream) {
+/* inner: NULL for fmb_gmres_transpose, else the adjoint solve refines
+ * around GMRES on that plan (fmb_gmres_refined_transpose) */
+static int adjoint_run(fmb_plan *pl, fmb_plan *inner, double inner_tol, const fmb_c64 *x,
+                       int nrhs, const double *dirs, const double *pts, int npts, int restart,
+                       double tol, int maxit, double *J, fmb_c64 *dJ_dsigma, double *dJ_dV,
+                       double *dJ_dvn, fmb_gmres_info *info, void *stream) {
   const int shape = dJ_dV || dJ_dvn;
   if (!pl || !pl->ready || !x || nrhs < 1 || nrhs > pl->nrhs_max || !pts ||
       npts < 1 || !(dJ_dsigma || shape) || restart < 1 || !(tol > 0) || maxit < 1)
@@ -87,9 +89,10 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
   if ((rc = fmbh_field_t(pl, d_p, nrhs, d_pts, npts, d_kt, d_st, stream))) goto done;
   /* A^T lambda = F^T u */
   if ((rc = fmbh_memset(d_lam, 0, sizeof(fmb_c64) * nv, stream))) goto done;
-  if ((rc = fmb_gmres_transpose(pl, d_kt, d_lam, nrhs, restart, tol, maxit, info,
-                                stream)))
-    goto done;
+  rc = inner ? fmb_gmres_refined_transpose(pl, inner, d_kt, d_lam, nrhs, restart, tol, inner_tol,
+                                           maxit, info, stream)
+             : fmb_gmres_transpose(pl, d_kt, d_lam, nrhs, restart, tol, maxit, info, stream);
+  if (rc) goto done;
   /* C^T lambda in FP64 with the sigma-derivative basis and near blocks */
   if ((rc = fmbh_malloc(&d_bs, sizeof(fmb_c64) * (size_t)N * pL * pL))) goto done;
   if ((rc = fmbh_malloc(&d_nc, sizeof(fmb_c64) * (pl->near_total ? pl->near_total : 1))))
@@ -127,4 +130,22 @@ done:
   fmbh_free(d_lx);
   fmbh_free(d_dv);
   return rc;
+}
+
+int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dirs,
+                     const double *pts, int npts, int restart, double tol, int maxit,
+                     double *J, fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
+                     fmb_gmres_info *info, void *stream) {
+  return adjoint_run(pl, NULL, 0.0, x, nrhs, dirs, pts, npts, restart, tol, maxit, J, dJ_dsigma,
+                     dJ_dV, dJ_dvn, info, stream);
+}
+
+int fmb_adjoint_grad_refined(fmb_plan *pl, fmb_plan *inner, double inner_tol,
+                             const fmb_c64 *x, int nrhs, const double *dirs, const double *pts,
+                             int npts, int restart, double tol, int maxit, double *J,
+                             fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
+                             fmb_gmres_info *info, void *stream) {
+  if (!inner) return fmb_set_error(FMB_ERR_ARG, "fmb_adjoint_grad_refined: no inner plan");
+  return adjoint_run(pl, inner, inner_tol, x, nrhs, dirs, pts, npts, restart, tol, maxit, J,
+                     dJ_dsigma, dJ_dV, dJ_dvn, info, stream);
 }

@@ -357,6 +357,14 @@ int fmb_adjoint_grad(fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *dir
                      const double *pts, int npts, int restart, double tol, int maxit,
                      double *J, fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
                      fmb_gmres_info *info, void *stream);
+/* fmb_adjoint_grad with the adjoint solve refined around GMRES on an inner
+ * plan (fmb_gmres_refined_transpose with inner_tol; same checks of the
+ * pair).  The reference has no adjoint. */
+int fmb_adjoint_grad_refined(fmb_plan *pl, fmb_plan *inner, double inner_tol,
+                             const fmb_c64 *x, int nrhs, const double *dirs, const double *pts,
+                             int npts, int restart, double tol, int maxit, double *J,
+                             fmb_c64 *dJ_dsigma, double *dJ_dV, double *dJ_dvn,
+                             fmb_gmres_info *info, void *stream);
 /* Precision of the Arnoldi matvecs of fmb_gmres on this plan (FMB_MIXED only
  * on a plan created with FMB_MIXED).  The reference has no solver. */
 int fmb_plan_set_work_precision(fmb_plan *pl, int prec);

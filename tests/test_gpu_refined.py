@@ -15,14 +15,16 @@ or 0.5, so its leaf order is lower.  Checks:
 * the transposed form (fmb_gmres_refined_transpose) against the transposed
   FP64 pass and fmb_gmres_transpose;
 * the batch over frequencies (fmb_gmres_batch_refined, graph and eager)
-  against each plan's FP64 residual and individual refined solves."""
+  against each plan's FP64 residual and individual refined solves;
+* adjoint gradients with the adjoint solve refined (fmb_adjoint_grad_refined)
+  against those of fmb_adjoint_grad."""
 import numpy as np
 import pytest
 import torch
 
 import fmbem_gpu_c_amd as F
 from conftest import wavenumber
-from test_oracle_adjoint import admittance
+from test_oracle_adjoint import admittance, circle_points
 
 pytestmark = pytest.mark.gpu
 
@@ -174,3 +176,30 @@ def test_batch_refined_matches_individual(graph):
         assert 1 <= infos[p].matvecs_fp64 <= infos[p].restarts
         assert err < 1e-7, (p, err)
         c += r
+
+
+def test_refined_adjoint_gradient(sphere):
+    """J, dJ/dsigma, dJ/dV and dJ/dvn with the adjoint solve refined around an
+    eps 0.5 inner plan against fmb_adjoint_grad (both adjoint solves to
+    1e-11): 1e-8 relative; J does not depend on the adjoint solve."""
+    m, t = sphere
+    k = wavenumber(FREQ)
+    n = m.arrays()[1].shape[0]
+    sig = torch.tensor(1j * k * admittance(n, 12), device="cuda")
+    pl = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED)
+    inner = F.Plan(t, k, robin=sig, precision=F.FMB_MIXED, eps=0.5)
+    d = np.array([[0.0, 0.0, 1.0]])
+    x, _ = pl.gmres(pl.rhs_plane_wave(d), tol=1e-11)
+    pts = circle_points()
+    J0, g0, i0, dV0, dvn0 = pl.adjoint_grad(x, pts, d, tol=1e-11, vertices=True)
+    J1, g1, i1, dV1, dvn1 = pl.adjoint_grad(x, pts, d, tol=1e-11, vertices=True, inner=inner)
+    print("adjoint: plain %d steps, refined %d cycles %d inner steps" % (i0.iters, i1.restarts, i1.iters),
+          flush=True)
+    assert i1.restarts >= 2
+    assert abs(J1 - J0) <= 1e-13 * abs(J0)
+    for a, b in ((g1, g0), (dV1, dV0), (dvn1, dvn0)):
+        err = (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item()
+        print("gradient rel diff %.2e" % err, flush=True)
+        assert err < 1e-8, err
+    with pytest.raises(F.FmbError, match="inner plan"):
+        pl.adjoint_grad(x, pts, d, inner=F.Plan(t, 1.01 * k, robin=sig, eps=0.5))
