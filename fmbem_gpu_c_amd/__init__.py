@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfmbem.so")
+# FMBEM_LIB selects another build of the same library (the sanitizer build)
+LIB_PATH = os.environ.get("FMBEM_LIB") or os.path.join(_HERE, "libfmbem.so")
 
 FMB_FP64, FMB_MIXED = 0, 1
 PRULE_TRUNCNUM, PRULE_EXCESS, PRULE_EXPLICIT = 0, 1, 2
@@ -181,6 +182,24 @@ def _dev(t, dtype=None):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _vecs(t, n, name, rows=None, like=None):
+    """Checks a tensor of right-hand-side vectors before its pointer reaches
+    the C ABI, which cannot see sizes: shape (n,) or (r, n) with r >= 1 (r
+    equal to rows when given, and the shape of `like` when given).  Returns
+    the number of vectors r.  Raised as FmbError(-1), before any device
+    access, so a mis-sized tensor never becomes an out-of-bounds kernel
+    access."""
+    shape = tuple(getattr(t, "shape", ()))
+    if len(shape) not in (1, 2) or shape[-1] != n or (len(shape) == 2 and shape[0] < 1):
+        raise FmbError(-1, "%s: expected shape (%d,) or (r, %d), got %s" % (name, n, n, shape))
+    r = 1 if len(shape) == 1 else shape[0]
+    if rows is not None and r != rows:
+        raise FmbError(-1, "%s: expected %d vectors, got %d" % (name, rows, r))
+    if like is not None and shape != tuple(like.shape):
+        raise FmbError(-1, "%s: shape %s differs from %s" % (name, shape, tuple(like.shape)))
+    return r
+
+
 def _stream():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -243,6 +262,9 @@ class Tree:
 
     def __init__(self, mesh, s=5, depth=0):
         self.mesh = mesh
+        nv, nf = ctypes.c_int(), ctypes.c_int()
+        _check(lib().fmb_mesh_sizes(mesh.h, ctypes.byref(nv), ctypes.byref(nf)))
+        self.n, self.nvert = nf.value, nv.value
         self.h = _vp()
         _check(lib().fmb_tree_build(mesh.h, s, depth, _stream(), ctypes.byref(self.h)))
 
@@ -268,13 +290,13 @@ class Tree:
 
     def vertex_normals(self):
         """Unit area-weighted vertex normals (nvert x 3)."""
-        nv = self.mesh.arrays()[0].shape[0]
+        nv = self.nvert
         out = np.zeros((nv, 3), dtype=np.float64)
         _check(lib().fmb_tree_vertex_normals(self.h, _ptr(out, ctypes.c_double)))
         return out
 
     def elem_box(self):
-        n = self.mesh.arrays()[1].shape[0]
+        n = self.n
         box = np.zeros(n, dtype=np.int32)
         perm = np.zeros(n, dtype=np.int32)
         _check(lib().fmb_tree_elem_box(self.h, _ptr(box, ctypes.c_int), _ptr(perm, ctypes.c_int)))
@@ -317,8 +339,12 @@ class Plan:
 
     def __init__(self, tree, k, robin=None, **kw):
         self.tree = tree
+        self.n = tree.n
         self.params = plan_params(k, **kw)
+        self.nrhs_max = self.params.nrhs_max
         self.h = _vp()
+        if robin is not None:
+            _vecs(robin, self.n, "robin", rows=1)
         rb = _dev(robin) if robin is not None else None
         _check(lib().fmb_plan_create(tree.h, ctypes.byref(self.params), rb, _stream(),
                                      ctypes.byref(self.h)))
@@ -326,6 +352,8 @@ class Plan:
     def retune(self, k, robin=None):
         """Re-targets the plan at wavenumber k (fmb_plan_retune); robin as
         for the constructor, at the new frequency."""
+        if robin is not None:
+            _vecs(robin, self.n, "robin", rows=1)
         rb = _dev(robin) if robin is not None else None
         _check(lib().fmb_plan_retune(self.h, float(k), rb, _stream()))
 
@@ -339,7 +367,7 @@ class Plan:
         """Initial guess at the plan's wavenumber from the solution x_from at
         k_from, carrying each right-hand side's incident phase."""
         import torch
-        nrhs = 1 if x_from.dim() == 1 else x_from.shape[0]
+        nrhs = self._nrhs(x_from, "x_from")
         d = None
         if dirs is not None:
             d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
@@ -367,6 +395,15 @@ class Plan:
                                          ctypes.byref(ms)))
         return tuple(x.value for x in v) + (ms.value,)
 
+    def _nrhs(self, t, name, like=None):
+        """Vectors in t (checked against the plan's element count and
+        nrhs_max)."""
+        r = _vecs(t, self.n, name, like=like)
+        if r > self.nrhs_max:
+            raise FmbError(-1, "%s: %d vectors above the plan's nrhs_max %d"
+                           % (name, r, self.nrhs_max))
+        return r
+
     def orders(self):
         p = np.zeros(16, dtype=np.int32)
         _check(lib().fmb_plan_orders(self.h, _ptr(p, ctypes.c_int)))
@@ -379,15 +416,16 @@ class Plan:
 
     def matvec(self, x, y=None, prec=FMB_FP64):
         import torch
-        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        nrhs = self._nrhs(x, "x")
         if y is None:
             y = torch.empty_like(x)
+        _vecs(y, self.n, "y", like=x)
         _check(lib().fmb_matvec(self.h, prec, _dev(x), _dev(y), nrhs, _stream()))
         return y
 
     def matvec_parts(self, x, parts, prec=FMB_FP64):
         import torch
-        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        nrhs = self._nrhs(x, "x")
         y = torch.empty_like(x)
         _check(lib().fmb_matvec_parts(self.h, prec, parts, _dev(x), _dev(y), nrhs, _stream()))
         return y
@@ -395,7 +433,10 @@ class Plan:
     def rhs_plane_wave(self, dirs):
         import torch
         dirs = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
-        n = self.tree.mesh.arrays()[1].shape[0]
+        if dirs.ndim != 2 or dirs.shape[1] != 3 or not 1 <= dirs.shape[0] <= self.nrhs_max:
+            raise FmbError(-1, "dirs: expected 1..%d unit directions (r, 3), got %s"
+                           % (self.nrhs_max, dirs.shape))
+        n = self.n
         b = torch.empty((dirs.shape[0], n), dtype=torch.complex128, device="cuda")
         _check(lib().fmb_rhs_plane_wave(self.h, _ptr(dirs, ctypes.c_double), dirs.shape[0],
                                         _dev(b), _stream()))
@@ -407,7 +448,7 @@ class Plan:
         plane waves (nrhs x 3) added to the scattered field, or None.
         Returns a [nrhs][M] complex128 CUDA tensor."""
         import torch
-        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        nrhs = _vecs(x, self.n, "x")
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         d = None
         if dirs is not None:
@@ -423,8 +464,10 @@ class Plan:
 
     def gmres(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
         import torch
+        nrhs = self._nrhs(b, "b")
+        if x0 is not None:
+            _vecs(x0, self.n, "x0", like=b)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
-        nrhs = 1 if b.dim() == 1 else b.shape[0]
         info = GmresInfo()
         rc = lib().fmb_gmres(self.h, _dev(b), _dev(x), nrhs, restart, tol, maxit,
                              ctypes.byref(info), _stream())
@@ -435,7 +478,7 @@ class Plan:
     def matvec_t(self, x, parts=3, prec=FMB_FP64):
         """y = A^T x (fmb_matvec_transpose)."""
         import torch
-        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        nrhs = self._nrhs(x, "x")
         y = torch.empty_like(x)
         _check(lib().fmb_matvec_transpose(self.h, prec, parts, _dev(x), _dev(y), nrhs,
                                           _stream()))
@@ -458,8 +501,13 @@ class Plan:
 
     def _refined(self, fn, inner, b, x0, restart, tol, inner_tol, maxit, check):
         import torch
+        if inner.n != self.n:
+            raise FmbError(-1, "inner plan of %d elements, outer of %d" % (inner.n, self.n))
+        nrhs = self._nrhs(b, "b")
+        inner._nrhs(b, "b")
+        if x0 is not None:
+            _vecs(x0, self.n, "x0", like=b)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
-        nrhs = 1 if b.dim() == 1 else b.shape[0]
         info = GmresInfo()
         rc = fn(self.h, inner.h, _dev(b), _dev(x), nrhs, restart, tol, inner_tol, maxit,
                 ctypes.byref(info), _stream())
@@ -470,8 +518,10 @@ class Plan:
     def gmres_t(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):
         """Solves A^T x = b (fmb_gmres_transpose)."""
         import torch
+        nrhs = self._nrhs(b, "b")
+        if x0 is not None:
+            _vecs(x0, self.n, "x0", like=b)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
-        nrhs = 1 if b.dim() == 1 else b.shape[0]
         info = GmresInfo()
         rc = lib().fmb_gmres_transpose(self.h, _dev(b), _dev(x), nrhs, restart, tol, maxit,
                                        ctypes.byref(info), _stream())
@@ -489,18 +539,20 @@ class Plan:
         displacement) (nvert) as float64 CUDA tensors.  With an inner plan the
         adjoint solve refines around GMRES on it (fmb_adjoint_grad_refined)."""
         import torch
-        nrhs = 1 if x.dim() == 1 else x.shape[0]
+        nrhs = self._nrhs(x, "x")
+        if inner is not None and inner.n != self.n:
+            raise FmbError(-1, "inner plan of %d elements, outer of %d" % (inner.n, self.n))
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         d = None
         if dirs is not None:
             d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
             if d.shape[0] != nrhs:
                 raise FmbError(-1, "one incident direction per right-hand side")
-        n = x.shape[-1]
+        n = self.n
         g = torch.empty(n, dtype=torch.complex128, device=x.device)
         dV = dvn = None
         if vertices:
-            nv = self.tree.mesh.arrays()[0].shape[0]
+            nv = self.tree.nvert
             dV = torch.empty((nv, 3), dtype=torch.float64, device=x.device)
             dvn = torch.empty(nv, dtype=torch.float64, device=x.device)
         J = ctypes.c_double()
@@ -556,11 +608,33 @@ def gen_coax_mats(kind, k, t, p):
     return out
 
 
+def _batch_rows(plans, nrhs, t, name):
+    """Checks the columns of a batch: one count per plan within its
+    nrhs_max, every plan on a mesh of the same element count, and t of
+    shape (sum(nrhs), N)."""
+    nrhs = [int(r) for r in nrhs]
+    if len(plans) < 1 or len(nrhs) != len(plans):
+        raise FmbError(-1, "one right-hand-side count per plan")
+    n = plans[0].n
+    for p, r in zip(plans, nrhs):
+        if p.n != n:
+            raise FmbError(-1, "batch plans on meshes of %d and %d elements" % (n, p.n))
+        if not 1 <= r <= p.nrhs_max:
+            raise FmbError(-1, "%d right-hand sides for a plan with nrhs_max %d" % (r, p.nrhs_max))
+    tot = sum(nrhs)
+    shape = tuple(getattr(t, "shape", ()))
+    if shape != (tot, n) and not (tot == 1 and shape == (n,)):
+        raise FmbError(-1, "%s: expected shape (%d, %d), got %s" % (name, tot, n, shape))
+    return nrhs
+
+
 def matvec_batch(plans, nrhs, x, graph=True, prec=FMB_FP64, reps=1):
     """fmb_matvec_batch: every plan applied to its columns of x
     ([sum(nrhs)][N]); returns [reps][sum(nrhs)][N] (one slice per replay)."""
     import torch
-    nrhs = [int(r) for r in nrhs]
+    nrhs = _batch_rows(plans, nrhs, x, "x")
+    if reps < 1:
+        raise FmbError(-1, "reps must be at least 1")
     arr = (_vp * len(plans))(*[p.h for p in plans])
     nr = (ctypes.c_int * len(plans))(*nrhs)
     y = torch.empty((reps,) + tuple(x.shape), dtype=x.dtype, device=x.device)
@@ -575,7 +649,9 @@ def gmres_batch(plans, nrhs, b, x0=None, graph=True, restart=30, tol=1e-6, maxit
     b: complex128 CUDA tensor [sum(nrhs)][N], plan 0's columns first.
     Returns (x, list of GmresInfo, one per plan)."""
     import torch
-    nrhs = [int(r) for r in nrhs]
+    nrhs = _batch_rows(plans, nrhs, b, "b")
+    if x0 is not None:
+        _batch_rows(plans, nrhs, x0, "x0")
     arr = (_vp * len(plans))(*[p.h for p in plans])
     nr = (ctypes.c_int * len(plans))(*nrhs)
     x = torch.zeros_like(b) if x0 is None else x0.clone()
@@ -593,11 +669,14 @@ def gmres_batch_refined(outers, inners, nrhs, b, x0=None, graph=True, restart=30
     around GMRES on the inner plans in lock step (inner p on outer p's tree,
     wavenumber and boundary condition).  b and the result as gmres_batch."""
     import torch
-    nrhs = [int(r) for r in nrhs]
-    no = (_vp * len(outers))(*[p.h for p in outers])
-    ni = (_vp * len(inners))(*[p.h for p in inners])
     if len(inners) != len(outers):
         raise FmbError(-1, "one inner plan per outer plan")
+    nrhs = _batch_rows(outers, nrhs, b, "b")
+    _batch_rows(inners, nrhs, b, "b")
+    if x0 is not None:
+        _batch_rows(outers, nrhs, x0, "x0")
+    no = (_vp * len(outers))(*[p.h for p in outers])
+    ni = (_vp * len(inners))(*[p.h for p in inners])
     nr = (ctypes.c_int * len(outers))(*nrhs)
     x = torch.zeros_like(b) if x0 is None else x0.clone()
     infos = (GmresInfo * len(outers))()
@@ -638,4 +717,5 @@ def _admittance_function():
 
 def admittance_objective(Y, plan, pts, dirs, tol=1e-10):
     """Differentiable J(Y) (torch.autograd) through the adjoint FMM-BEM."""
+    _vecs(Y, plan.n, "Y", rows=1)
     return _admittance_function().apply(Y, plan, pts, dirs, tol)

@@ -125,6 +125,14 @@ static int sys_init(sys_t *S, fmb_plan *const *pls, const int *nrs, int np, int 
       return fmb_set_error(FMB_ERR_ARG, "fmb_gmres: plan %d does not fit the batch", p);
     if (trans && q->prm.rcr_path == FMB_RCR_VALU)
       return fmb_set_error(FMB_ERR_ARG, "transposed solve needs rcr_path FMB_RCR_AUTO");
+    /* the branches of a captured batch run concurrently, and a plan's pass
+     * owns its staging vectors, coefficients and workspace: a plan listed
+     * twice would race with itself */
+    for (int r = 0; graph && r < p; r++)
+      if (pls[r] == q)
+        return fmb_set_error(FMB_ERR_ARG,
+                             "fmb_gmres: plan %d is plan %d again (a graph batch needs "
+                             "distinct plans)", p, r);
   }
   S->pl = pls;
   S->nr = nrs;
@@ -257,11 +265,24 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
   if ((rc = fmbh_znorms(x, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
   int x_zero = 1;
-  for (int r = 0; r < nrhs; r++) x_zero &= bn[r] == 0.0;
+  for (int r = 0; r < nrhs; r++) {
+    /* a NaN compares false with every tolerance and would read as
+     * converged: non-finite inputs are refused */
+    if (!isfinite(bn[r])) {
+      rc = fmb_set_error(FMB_ERR_ARG, "fmb_gmres: initial guess %d holds NaN or Inf", r);
+      goto done;
+    }
+    x_zero &= bn[r] == 0.0;
+  }
   if ((rc = fmbh_znorms(b, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
-  for (int r = 0; r < nrhs; r++)
+  for (int r = 0; r < nrhs; r++) {
+    if (!isfinite(bn[r])) {
+      rc = fmb_set_error(FMB_ERR_ARG, "fmb_gmres: right-hand side %d holds NaN or Inf", r);
+      goto done;
+    }
     if (bn[r] == 0.0) bn[r] = 1.0;
+  }
   while (1) {
     /* true residual r = b - A x in FP64 (A x = 0 for a zero first guess) */
     if (x_zero) {
@@ -294,6 +315,12 @@ static int gmres_run(fmb_plan *const *pls, const int *nrs, int np, int trans, in
         for (int r = S.off[p]; r < S.off[p + 1]; r++) rn[r] = rsave[r];
     conv = 1;
     for (int r = 0; r < nrhs; r++) {
+      if (!isfinite(rn[r])) {
+        rc = fmb_set_error(FMB_ERR_NOCONV,
+                           "GMRES: true residual of right-hand side %d is not finite "
+                           "after %d steps", r, iters);
+        goto done;
+      }
       active[r] = rn[r] / bn[r] > tol;
       if (active[r]) conv = 0;
     }
@@ -607,11 +634,23 @@ static int refined_run(fmb_plan *const *outers, fmb_plan *const *inners, const i
   if ((rc = fmbh_h2d(dh2, mone, sizeof(fmb_c64) * nrhs, stream))) goto done;
   if ((rc = fmbh_znorms(x, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(rn, dn, sizeof(double) * nrhs, stream))) goto done;
-  for (int r = 0; r < nrhs; r++) x_zero &= rn[r] == 0.0;
+  for (int r = 0; r < nrhs; r++) {
+    if (!isfinite(rn[r])) {
+      rc = fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: initial guess %d holds NaN or Inf", r);
+      goto done;
+    }
+    x_zero &= rn[r] == 0.0;
+  }
   if ((rc = fmbh_znorms(b, nrhs, N, dn, stream))) goto done;
   if ((rc = fmbh_d2h(bn, dn, sizeof(double) * nrhs, stream))) goto done;
-  for (int r = 0; r < nrhs; r++)
+  for (int r = 0; r < nrhs; r++) {
+    if (!isfinite(bn[r])) {
+      rc = fmb_set_error(FMB_ERR_ARG, "fmb_gmres_refined: right-hand side %d holds NaN or Inf",
+                         r);
+      goto done;
+    }
     if (bn[r] == 0.0) bn[r] = 1.0;
+  }
   while (1) {
     /* r = b - A_out x in FP64 (r = b for a zero first guess); converged
      * plans keep r = 0 */
@@ -634,6 +673,13 @@ static int refined_run(fmb_plan *const *outers, fmb_plan *const *inners, const i
       if (!pdone[p])
         for (int r = off[p]; r < off[p + 1]; r++) rn[r] = tn[r];
     conv = 1;
+    for (int r = 0; r < nrhs; r++)
+      if (!isfinite(rn[r])) {
+        rc = fmb_set_error(FMB_ERR_NOCONV,
+                           "fmb_gmres_refined: residual of right-hand side %d is not finite "
+                           "after %d inner steps", r, iters);
+        goto done;
+      }
     for (int p = 0; p < np; p++) {
       int ok = 1;
       for (int r = off[p]; r < off[p + 1]; r++) ok &= rn[r] / bn[r] <= tol;

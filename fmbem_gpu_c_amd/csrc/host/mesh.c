@@ -68,12 +68,28 @@ static int is_rec(const char *line, char c) {
   return line[0] == c && (line[1] == ' ' || line[1] == '\t');
 }
 
+/* fgets that rejects a line longer than the buffer (a split line would be
+ * read as two records); returns 1 for a line, 0 at end of file, -1 for an
+ * overlong line */
+static int read_line(char *buf, int size, FILE *fp) {
+  if (!fgets(buf, size, fp)) return 0;
+  size_t n = strlen(buf);
+  if (n == (size_t)size - 1 && buf[n - 1] != '\n' && !feof(fp)) return -1;
+  return 1;
+}
+
+#define OBJ_LINE 4096
+#define OBJ_MAX_POLY 256 /* vertices of one polygon record */
+
 int fmb_obj_count(const char *path, int *nv, int *nf) {
+  if (!path || !nv || !nf) return fmb_set_error(FMB_ERR_ARG, "fmb_obj_count: null argument");
   FILE *fp = fopen(path, "r");
   if (!fp) return fmb_set_error(FMB_ERR_IO, "cannot open %s", path);
-  char line[4096];
-  int v = 0, f = 0;
-  while (fgets(line, sizeof(line), fp)) {
+  char line[OBJ_LINE];
+  long long v = 0, f = 0;
+  int lno = 0, r;
+  while ((r = read_line(line, sizeof(line), fp)) > 0) {
+    lno++;
     if (is_rec(line, 'v')) v++;
     if (is_rec(line, 'f')) {
       /* polygons are fan-triangulated */
@@ -85,16 +101,34 @@ int fmb_obj_count(const char *path, int *nv, int *nf) {
         nt++;
         while (*p && !isspace((unsigned char)*p)) p++;
       }
+      if (nt > OBJ_MAX_POLY) {
+        fclose(fp);
+        return fmb_set_error(FMB_ERR_IO, "%s:%d: polygon of %d vertices (at most %d)", path,
+                             lno, nt, OBJ_MAX_POLY);
+      }
       if (nt >= 3) f += nt - 2;
     }
   }
   fclose(fp);
-  *nv = v;
-  *nf = f;
+  if (r < 0) return fmb_set_error(FMB_ERR_IO, "%s:%d: line longer than %d bytes", path,
+                                  lno + 1, OBJ_LINE - 1);
+  if (v > 0x7fffffff / 3 || f > 0x7fffffff / 3)
+    return fmb_set_error(FMB_ERR_IO, "%s: too many records", path);
+  *nv = (int)v;
+  *nf = (int)f;
   return FMB_OK;
 }
 
+/* fails the load: closes the file, frees the mesh, records the message */
+#define OBJ_FAIL(...)                                  \
+  do {                                                 \
+    fclose(fp);                                        \
+    fmb_mesh_destroy(m);                               \
+    return fmb_set_error(FMB_ERR_IO, __VA_ARGS__);     \
+  } while (0)
+
 int fmb_mesh_load_obj(const char *path, fmb_mesh **out) {
+  if (!out) return fmb_set_error(FMB_ERR_ARG, "fmb_mesh_load_obj: null output");
   int nv, nf, rc = fmb_obj_count(path, &nv, &nf);
   if (rc) return rc;
   if (nv < 3 || nf < 1)
@@ -106,38 +140,40 @@ int fmb_mesh_load_obj(const char *path, fmb_mesh **out) {
     fmb_mesh_destroy(m);
     return fmb_set_error(FMB_ERR_IO, "cannot open %s", path);
   }
-  char line[4096];
-  int iv = 0, jf = 0, lno = 0;
-  while (fgets(line, sizeof(line), fp)) {
+  char line[OBJ_LINE];
+  int iv = 0, jf = 0, lno = 0, r;
+  while ((r = read_line(line, sizeof(line), fp)) > 0) {
     lno++;
     if (is_rec(line, 'v')) {
       double x, y, z;
-      if (sscanf(line + 1, "%lf %lf %lf", &x, &y, &z) != 3) {
-        fclose(fp);
-        fmb_mesh_destroy(m);
-        return fmb_set_error(FMB_ERR_IO, "%s:%d: malformed vertex", path, lno);
-      }
+      if (iv >= nv) OBJ_FAIL("%s:%d: more vertices than counted (file changed)", path, lno);
+      if (sscanf(line + 1, "%lf %lf %lf", &x, &y, &z) != 3)
+        OBJ_FAIL("%s:%d: malformed vertex", path, lno);
       m->V[3 * iv] = x;
       m->V[3 * iv + 1] = y;
       m->V[3 * iv + 2] = z;
       iv++;
     } else if (is_rec(line, 'f')) {
-      int ids[256], n = 0;
+      int ids[OBJ_MAX_POLY], n = 0;
       char *p = line + 1;
-      while (*p && n < 256) {
+      while (*p) {
         while (*p && isspace((unsigned char)*p)) p++;
         if (!*p) break;
+        if (n == OBJ_MAX_POLY)
+          OBJ_FAIL("%s:%d: polygon of more than %d vertices", path, lno, OBJ_MAX_POLY);
         char *end;
         long v = strtol(p, &end, 10);
-        if (end == p) {
-          fclose(fp);
-          fmb_mesh_destroy(m);
-          return fmb_set_error(FMB_ERR_IO, "%s:%d: malformed face", path, lno);
-        }
-        /* 1-based ids (readOBJ, octree.c:606-609); negative = relative */
-        ids[n++] = v > 0 ? (int)v - 1 : iv + (int)v;
+        /* 1-based ids (readOBJ, octree.c:606-609); negative = relative to
+         * the vertices read so far; 0 is not an id */
+        if (end == p || v == 0) OBJ_FAIL("%s:%d: malformed face", path, lno);
+        long id = v > 0 ? v - 1 : iv + v;
+        if (id < 0 || id >= nv)
+          OBJ_FAIL("%s:%d: face references vertex %ld of %d", path, lno, v, nv);
+        ids[n++] = (int)id;
         while (*p && !isspace((unsigned char)*p)) p++; /* skip /vt/vn */
       }
+      if (n >= 3 && jf + (n - 2) > nf)
+        OBJ_FAIL("%s:%d: more faces than counted (file changed)", path, lno);
       for (int t = 1; t + 1 < n; t++) {
         m->F[3 * jf] = ids[0];
         m->F[3 * jf + 1] = ids[t];
@@ -146,17 +182,15 @@ int fmb_mesh_load_obj(const char *path, fmb_mesh **out) {
       }
     }
   }
+  if (r < 0) OBJ_FAIL("%s:%d: line longer than %d bytes", path, lno + 1, OBJ_LINE - 1);
+  if (iv != nv || jf != nf)
+    OBJ_FAIL("%s: %d vertices and %d faces read, %d and %d counted (file changed)", path, iv,
+             jf, nv, nf);
   fclose(fp);
-  for (int i = 0; i < 3 * nf; i++)
-    if (m->F[i] < 0 || m->F[i] >= nv) {
-      int bad = m->F[i];
-      fmb_mesh_destroy(m);
-      return fmb_set_error(FMB_ERR_IO, "%s: face references vertex %d of %d",
-                           path, bad + 1, nv);
-    }
   *out = m;
   return FMB_OK;
 }
+#undef OBJ_FAIL
 
 /* signed volume with the reference normal (n0-n2)x(n1-n2) */
 static double signed_volume(const fmb_mesh *m) {

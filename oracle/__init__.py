@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects the sanitizer build (make -C oracle san)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 RULE_COLLAPSED, RULE_GAUSS7 = 0, 1
 COAX_RR, COAX_SR = 0, 1

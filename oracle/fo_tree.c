@@ -190,10 +190,16 @@ static int cmp_kv(const void *a, const void *b) {
 int fo_tree_build(const double *V, int nv, const int *F, int nf, int s,
                   int depth, fo_tree **out) {
   (void)nv;
+  if (nf < 1) return -1;
   fo_tree *t = (fo_tree *)calloc(1, sizeof(fo_tree));
+  double *ctr = (double *)malloc(sizeof(double) * 3 * nf);
+  if (!t || !ctr) {
+    free(t);
+    free(ctr);
+    return -1;
+  }
   t->nelem = nf;
   t->lmin = 2;
-  double *ctr = (double *)malloc(sizeof(double) * 3 * nf);
   for (int i = 0; i < nf; i++) {
     const double *a = V + 3 * F[3 * i], *b = V + 3 * F[3 * i + 1],
                  *c = V + 3 * F[3 * i + 2];

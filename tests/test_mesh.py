@@ -92,3 +92,48 @@ def test_perturbed_ellipsoid():
 def test_mesh_from_arrays_validates():
     with pytest.raises(F.FmbError):
         F.Mesh.from_arrays(np.zeros((3, 3)), np.array([[0, 1, 5]]))
+
+
+def _obj_error(path):
+    with pytest.raises(F.FmbError) as e:
+        F.Mesh.from_obj(str(path))
+    assert e.value.code == -2
+    return str(e.value)
+
+
+def test_obj_large_polygon_is_refused_not_truncated(tmp_path):
+    """A 300-gon exceeds the 256-vertex polygon limit: an error, never a
+    partly filled face array."""
+    n = 300
+    th = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    lines = ["v %.6f %.6f 0" % (np.cos(t), np.sin(t)) for t in th]
+    lines.append("f " + " ".join(str(i + 1) for i in range(n)))
+    p = tmp_path / "poly300.obj"
+    p.write_text("\n".join(lines) + "\n")
+    assert "polygon" in _obj_error(p)
+    # 256 vertices is the largest polygon accepted (254 fan triangles)
+    lines = lines[:256] + ["f " + " ".join(str(i + 1) for i in range(256))]
+    p.write_text("\n".join(lines) + "\n")
+    V, Fc = F.Mesh.from_obj(str(p)).arrays()
+    assert Fc.shape == (254, 3) and Fc.min() == 0 and Fc.max() == 255
+
+
+def test_obj_truncated_and_malformed_files(tmp_path):
+    good = "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\nf 1 2 4\n"
+    p = tmp_path / "t.obj"
+    # cut inside the last face record: its third id is missing
+    p.write_text(good[:-3])
+    F.Mesh.from_obj(str(p))  # "f 1 2" is a degenerate record, skipped
+    p.write_text(good[:-8] + "f 1 2 7\n")
+    assert "vertex" in _obj_error(p)
+    p.write_text(good.replace("f 1 2 4", "f 1 0 4"))  # ids are 1-based
+    assert "malformed" in _obj_error(p)
+    p.write_text(good.replace("f 1 2 4", "f 1 -9 4"))  # relative id before the first vertex
+    _obj_error(p)
+    p.write_text("v 0 0 0\nv 1 0\nv 0 1 0\nf 1 2 3\n")
+    assert "malformed vertex" in _obj_error(p)
+    p.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3 " + "1 " * 3000 + "\n")
+    assert "longer" in _obj_error(p)
+    p.write_text(good.replace("f 1 2 4", "f -4 -3 -1"))  # relative ids
+    V, Fc = F.Mesh.from_obj(str(p)).arrays()
+    assert Fc.tolist() == [[0, 1, 2], [0, 1, 3]]

@@ -125,3 +125,16 @@ def test_trunc_num_closed_form():
     D = 1.0
     assert [O.trunc_num(2 * np.pi * 1e4 / 343.0, 0.05, 1.5, D / 2 ** l) for l in range(2, 8)] == \
         [94, 62, 42, 31, 23, 18]
+
+
+def test_deter_lmax_sentinel_for_coincident_points():
+    """More than s points in one level-10 box: the reference's deterLmax
+    returns 11 (octree.c:662-665), which the device build maps to
+    FMB_ERR_DEPTH; s points or fewer still give a depth."""
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(0, 1, (50, 3))
+    dup = np.vstack([pts, np.repeat(pts[:1], 5, axis=0)])  # 6 copies of one point
+    assert O.deter_lmax(dup, 5) == 11
+    assert O.deter_lmax(dup, 6) <= 10
+    ok = np.vstack([pts, np.repeat(pts[:1], 4, axis=0)])  # 5 copies
+    assert O.deter_lmax(ok, 5) <= 10
