@@ -100,6 +100,8 @@ _SIGS = {
     "fmb_gen_rot_mats": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _vp]),
     "fmb_gen_coax_mats": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int,
                                          ctypes.c_int, _vp]),
+    "fmb_reduce_rot_mat": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int]),
+    "fmb_reduce_coax_mat": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int]),
     "fmb_plan_params_default": (None, [ctypes.POINTER(PlanParams)]),
     "fmb_plan_create": (ctypes.c_int, [_vp, ctypes.POINTER(PlanParams), _vp, _vp, ctypes.POINTER(_vp)]),
     "fmb_plan_orders": (ctypes.c_int, [_vp, _ip]),
@@ -626,6 +628,32 @@ def _batch_rows(plans, nrhs, t, name):
     if shape != (tot, n) and not (tot == 1 and shape == (n,)):
         raise FmbError(-1, "%s: expected shape (%d, %d), got %s" % (name, tot, n, shape))
     return nrhs
+
+
+def reduce_rot_mat(mat, p1, p2):
+    """A rotation matrix of order p1 (reference layout) truncated to order p2
+    (fmb_reduce_rot_mat)."""
+    a = np.ascontiguousarray(mat, dtype=np.complex128)
+    if a.shape != (lib().fmb_rot_size(p1),):
+        raise FmbError(-1, "expected %d values of an order-%d rotation matrix"
+                       % (lib().fmb_rot_size(p1), p1))
+    out = np.zeros(lib().fmb_rot_size(max(p2, 1)), dtype=np.complex128)
+    _check(lib().fmb_reduce_rot_mat(a.ctypes.data_as(ctypes.c_void_p), p1,
+                                    out.ctypes.data_as(ctypes.c_void_p), p2))
+    return out
+
+
+def reduce_coax_mat(mat, p1, p2):
+    """A coaxial matrix of order p1 (|m|-block layout) truncated to order p2
+    (fmb_reduce_coax_mat)."""
+    a = np.ascontiguousarray(mat, dtype=np.complex128)
+    if a.shape != (lib().fmb_coax_size(p1),):
+        raise FmbError(-1, "expected %d values of an order-%d coaxial matrix"
+                       % (lib().fmb_coax_size(p1), p1))
+    out = np.zeros(lib().fmb_coax_size(max(p2, 1)), dtype=np.complex128)
+    _check(lib().fmb_reduce_coax_mat(a.ctypes.data_as(ctypes.c_void_p), p1,
+                                     out.ctypes.data_as(ctypes.c_void_p), p2))
+    return out
 
 
 def matvec_batch(plans, nrhs, x, graph=True, prec=FMB_FP64, reps=1):

@@ -1,8 +1,11 @@
 /* ops.c -- standalone operator generation in the reference layouts:
  * genSparseRotMats (translation.cu:1297-1394) and
  * gen{RR,SR}SparseCoaxTransMat (translation.cu:822-903, 982-1062), built on
- * the device in FP64 and copied to the caller's host buffer. */
+ * the device in FP64 and copied to the caller's host buffer; and their
+ * truncation to a lower order (reduceSparseRotMat / reduceSparseCoaxTransMat,
+ * translation.cu:22-56) on the host. */
 #include <stdlib.h>
+#include <string.h>
 
 #include "../fmb_internal.h"
 #include "../hip/fmb_hip.h"
@@ -44,4 +47,37 @@ int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
     rc = fmbh_d2h(out, d_out, sizeof(fmb_c64) * cs * n, NULL);
   fmbh_free(d_out);
   return rc;
+}
+
+/* reduceSparseRotMat (translation.cu:22-26): the n-blocks n < p2 of a
+ * rotation matrix of order p1 are its first fmb_rot_size(p2) entries. */
+int fmb_reduce_rot_mat(const fmb_c64 *in, int p1, fmb_c64 *out, int p2) {
+  if (!in || !out || p2 < 1 || p2 > p1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_reduce_rot_mat: orders %d -> %d", p1, p2);
+  memmove(out, in, sizeof(fmb_c64) * (size_t)fmb_rot_size(p2));
+  return FMB_OK;
+}
+
+/* offset of the |m|-block of a coaxial matrix of order p (sparseCoaxMatBlockIdx,
+ * translation.cu:28-32): blocks (p - m)^2 for m = 0, 1, ... */
+static size_t coax_block(int p, int m) {
+  return (size_t)m * (2 * (size_t)m * m - (3 + 6 * (size_t)p) * m + 1 + 6 * ((size_t)p + (size_t)p * p)) / 6;
+}
+
+/* reduceSparseCoaxTransMat (translation.cu:34-56): block m < p2 of the
+ * order-p2 matrix is the leading (p2-m) x (p2-m) part of block m of the
+ * order-p1 matrix, both column-major. */
+int fmb_reduce_coax_mat(const fmb_c64 *in, int p1, fmb_c64 *out, int p2) {
+  if (!in || !out || p2 < 1 || p2 > p1)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_reduce_coax_mat: orders %d -> %d", p1, p2);
+  if ((const void *)in == (const void *)out && p1 != p2)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_reduce_coax_mat: out must not alias in");
+  for (int m = 0; m < p2; m++) {
+    const fmb_c64 *a = in + coax_block(p1, m);
+    fmb_c64 *b = out + coax_block(p2, m);
+    const int l1 = p1 - m, l2 = p2 - m;
+    for (int c = 0; c < l2; c++)
+      for (int r = 0; r < l2; r++) b[(size_t)c * l2 + r] = a[(size_t)c * l1 + r];
+  }
+  return FMB_OK;
 }

@@ -116,6 +116,14 @@ int fmb_gen_rot_mats(const double *angles, int n, int p, fmb_c64 *out);
 #define FMB_COAX_SR 1
 int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
                       fmb_c64 *out);
+/* reduceSparseRotMat (translation.cu:22-26; translation.h:132): a rotation
+ * matrix of order p1 (host, reference n-block layout) truncated to order
+ * p2 <= p1: its first fmb_rot_size(p2) values (in and out may alias). */
+int fmb_reduce_rot_mat(const fmb_c64 *in, int p1, fmb_c64 *out, int p2);
+/* reduceSparseCoaxTransMat (translation.cu:34-56; translation.h:134): a coaxial
+ * matrix of order p1 (host, |m|-block layout) truncated to order p2 <= p1:
+ * block m < p2 keeps its leading (p2-m) x (p2-m) entries. */
+int fmb_reduce_coax_mat(const fmb_c64 *in, int p1, fmb_c64 *out, int p2);
 
 /* ------------------------------------------------------------ plan */
 #define FMB_FP64 0  /* plain path: FP64 everywhere                       */

@@ -138,3 +138,52 @@ def test_deter_lmax_sentinel_for_coincident_points():
     assert O.deter_lmax(dup, 6) <= 10
     ok = np.vstack([pts, np.repeat(pts[:1], 4, axis=0)])  # 5 copies
     assert O.deter_lmax(ok, 5) <= 10
+
+
+def test_numpy_index_sets_match_oracle():
+    """tests/treeref.py (used for the full-size device tree checks) against
+    the oracle: Morton round trip and box centres, I4 of sampled boxes at
+    levels 2-6, and the pair set of a random level set = I4 restricted to
+    it, with offset codes."""
+    import treeref as R
+    rng = np.random.default_rng(9)
+    for l in range(2, 7):
+        keys = rng.choice(8 ** l, size=min(40, 8 ** l), replace=False)
+        ijk = R.morton_decode(keys, l)
+        np.testing.assert_array_equal(R.morton_encode(ijk, l), keys)
+        for k, c in zip(keys[:10], ijk[:10]):
+            np.testing.assert_allclose(O.box_center(int(k), l), (c + 0.5) / 2 ** l, rtol=0,
+                                       atol=1e-15)
+            np.testing.assert_array_equal(R.i4_keys(int(k), l), np.sort(O.I4(int(k), l)))
+    l = 4
+    keys = np.sort(rng.choice(8 ** l, size=900, replace=False))
+    tg, sr, oc = R.m2l_pairs(keys, l)
+    kset = set(keys.tolist())
+    want = {(int(b), int(s)) for b in keys for s in O.I4(int(b), l) if int(s) in kset}
+    assert set(zip(tg.tolist(), sr.tolist())) == want
+    d = R.morton_decode(tg, l) - R.morton_decode(sr, l)
+    np.testing.assert_array_equal((d[:, 0] + 3) * 49 + (d[:, 1] + 3) * 7 + d[:, 2] + 3, oc)
+
+
+def test_reduce_matches_generation_at_lower_order():
+    """reduceSparseCoaxTransMat / reduceSparseRotMat (translation.cu:22-56)
+    as exported by the library, on the oracle's matrices: main.c's case
+    ((S|R), k = 13.1, t = 2.3, p = 5 reduced to 2, main.c:27-38) and wider
+    ones equal the matrices generated at the lower order; the layout is
+    checked entry by entry against the oracle's block offsets."""
+    import fmbem_gpu_c_amd as F
+    for kind, k, t, p1, p2 in [(O.COAX_SR, 13.1, 2.3, 5, 2), (O.COAX_SR, 18.3, 0.1, 16, 9),
+                               (O.COAX_RR, 91.6, 0.05, 30, 17)]:
+        big = O.coax_sparse(kind, p1, k, t)
+        got = F.reduce_coax_mat(big, p1, p2)
+        want = O.coax_sparse(kind, p2, k, t)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+        for m in range(p2):
+            a, b = O.lib().fo_coax_block(p1, m), O.lib().fo_coax_block(p2, m)
+            A = big[a:a + (p1 - m) ** 2].reshape(p1 - m, p1 - m)
+            B = got[b:b + (p2 - m) ** 2].reshape(p2 - m, p2 - m)
+            np.testing.assert_array_equal(A[:p2 - m, :p2 - m], B)
+    r = O.rot_sparse(12, 0.3, 1.1, -0.7)
+    np.testing.assert_array_equal(F.reduce_rot_mat(r, 12, 7), O.rot_sparse(7, 0.3, 1.1, -0.7))
+    with pytest.raises(F.FmbError):
+        F.reduce_coax_mat(big, p1, p1 + 1)
