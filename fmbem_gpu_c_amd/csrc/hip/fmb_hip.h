@@ -16,6 +16,7 @@ extern "C" {
 
 /* ---- runtime */
 int fmbh_init(void);
+int fmbh_device_tag(char *buf, int n);
 int fmbh_malloc(void **p, size_t bytes);
 int fmbh_free(void *p);
 int fmbh_h2d(void *dst, const void *src, size_t bytes, void *stream);

@@ -11,6 +11,21 @@ extern "C" int fmbh_init(void) {
   return FMB_OK;
 }
 
+// identity of the current device for keyed caches: architecture and CU count
+extern "C" int fmbh_device_tag(char *buf, int n) {
+  int dev = 0;
+  FMB_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  FMB_HIP_CHECK(hipGetDeviceProperties(&pr, dev));
+  char arch[64];
+  int i = 0;
+  // the architecture name up to its feature suffix (gfx950:sramecc+:xnack-)
+  for (; i < 63 && pr.gcnArchName[i] && pr.gcnArchName[i] != ':'; i++) arch[i] = pr.gcnArchName[i];
+  arch[i] = 0;
+  snprintf(buf, n, "%s/%dcu", arch, pr.multiProcessorCount);
+  return FMB_OK;
+}
+
 extern "C" int fmbh_malloc(void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;

@@ -16,8 +16,11 @@
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/file.h>
+#include <sys/stat.h>
 
 #include "../fmb_internal.h"
 #include "../hip/fmb_hip.h"
@@ -437,38 +440,136 @@ static int time_run(fmb_plan *pl, int op, int kind, int l, void *stream, float *
   return rc;
 }
 
-/* Shapes already tuned in this process, keyed by what the timing depends
- * on: precision, kind, orders, chunk count and right-hand sides (a sweep
- * that revisits a level's order reuses its result). */
+/* Shapes already tuned, keyed by what the timing depends on: device,
+ * precision, kind, orders, chunk count and right-hand sides (a sweep that
+ * revisits a level's order reuses its result).  The cache lives for the
+ * process and in a text file shared by processes: FMBEM_TUNE_CACHE names it
+ * (empty: no file), else $XDG_CACHE_HOME/fmbem/rcr_tiles.txt or
+ * $HOME/.cache/fmbem/rcr_tiles.txt.  Each tuned shape is appended as one
+ * line under an advisory lock; a process reads the file once, at its first
+ * autotuned plan, and a later line for the same key wins. */
 typedef struct {
   int prec, kind, pin, pout, nchunks, nrhs, bf;
   shape_t shape;
   float ms;
 } tuned_t;
-#define TUNE_CACHE 256
+#define TUNE_CACHE 4096
+#define TUNE_MAGIC "fmbem-rcr-tile-v1"
 static tuned_t tune_cache[TUNE_CACHE];
-static int tune_cached;
+static int tune_cached, tune_loaded;
+static char tune_dev[64];
 static pthread_mutex_t tune_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static int same_key(const tuned_t *c, const tuned_t *key) {
+  return c->prec == key->prec && c->kind == key->kind && c->pin == key->pin &&
+         c->pout == key->pout && c->nchunks == key->nchunks && c->nrhs == key->nrhs &&
+         c->bf == key->bf;
+}
+
+/* caller holds tune_lock: replaces an entry of the same key or appends */
+static void cache_store(const tuned_t *v) {
+  for (int i = 0; i < tune_cached; i++)
+    if (same_key(&tune_cache[i], v)) {
+      tune_cache[i] = *v;
+      return;
+    }
+  tune_cache[tune_cached < TUNE_CACHE ? tune_cached++ : TUNE_CACHE - 1] = *v;
+}
+
+/* the cache file's path, or 0 when there is none */
+static int tune_path(char *buf, size_t n) {
+  const char *e = getenv("FMBEM_TUNE_CACHE");
+  if (e) {
+    if (!*e || strlen(e) >= n) return 0;
+    strcpy(buf, e);
+    return 1;
+  }
+  const char *x = getenv("XDG_CACHE_HOME"), *h = getenv("HOME");
+  int w;
+  if (x && *x)
+    w = snprintf(buf, n, "%s/fmbem/rcr_tiles.txt", x);
+  else if (h && *h)
+    w = snprintf(buf, n, "%s/.cache/fmbem/rcr_tiles.txt", h);
+  else
+    return 0;
+  return w > 0 && (size_t)w < n;
+}
+
+/* creates the directories leading to path (mkdir -p of its dirname) */
+static void make_parents(const char *path) {
+  char d[1024];
+  size_t n = strlen(path);
+  if (n >= sizeof(d)) return;
+  memcpy(d, path, n + 1);
+  for (char *p = d + 1; *p; p++)
+    if (*p == '/') {
+      *p = 0;
+      mkdir(d, 0755);
+      *p = '/';
+    }
+}
+
+/* reads the file's entries for this device (once per process) */
+static void tune_load(void) {
+  if (tune_loaded) return;
+  tune_loaded = 1;
+  if (fmbh_device_tag(tune_dev, sizeof(tune_dev))) tune_dev[0] = 0;
+  char path[1024], line[512], magic[32], dev[64];
+  if (!tune_path(path, sizeof(path))) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  flock(fileno(f), LOCK_SH);
+  while (fgets(line, sizeof(line), f)) {
+    tuned_t v;
+    if (sscanf(line, "%31s %63s %d %d %d %d %d %d %d %d %d %d %d %d %f", magic, dev, &v.prec,
+               &v.kind, &v.pin, &v.pout, &v.nchunks, &v.nrhs, &v.bf, &v.shape.mode,
+               &v.shape.nt, &v.shape.mt, &v.shape.nw, &v.shape.bf, &v.ms) != 15)
+      continue;
+    if (strcmp(magic, TUNE_MAGIC) || strcmp(dev, tune_dev)) continue;
+    /* only shapes this build can launch; split-BF16 only where allowed */
+    if (v.shape.mode < FMBH_RCR_LDS || v.shape.mode > FMBH_RCR_LDSP || v.shape.nt < 1 ||
+        v.shape.nt > 2 || (v.shape.nw != 4 && v.shape.nw != 8) || v.kind < 0 || v.kind > 2 ||
+        (v.shape.mt != 2 && v.shape.mt != 3 && v.shape.mt != 4 && v.shape.mt != 6) ||
+        (v.shape.bf && (!v.bf || v.shape.mode == FMBH_RCR_GWS)) || !(v.ms >= 0))
+      continue;
+    cache_store(&v);
+  }
+  flock(fileno(f), LOCK_UN);
+  fclose(f);
+}
+
+static void tune_append(const tuned_t *v) {
+  char path[1024];
+  if (!tune_dev[0] || !tune_path(path, sizeof(path))) return;
+  make_parents(path);
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  flock(fileno(f), LOCK_EX);
+  fprintf(f, "%s %s %d %d %d %d %d %d %d %d %d %d %d %d %.4f\n", TUNE_MAGIC, tune_dev,
+          v->prec, v->kind, v->pin, v->pout, v->nchunks, v->nrhs, v->bf, v->shape.mode,
+          v->shape.nt, v->shape.mt, v->shape.nw, v->shape.bf, v->ms);
+  fflush(f);
+  flock(fileno(f), LOCK_UN);
+  fclose(f);
+}
 
 static int cache_find(const tuned_t *key, tuned_t *out) {
   int hit = 0;
   pthread_mutex_lock(&tune_lock);
-  for (int i = 0; i < tune_cached && !hit; i++) {
-    const tuned_t *c = &tune_cache[i];
-    if (c->prec == key->prec && c->kind == key->kind && c->pin == key->pin &&
-        c->pout == key->pout && c->nchunks == key->nchunks && c->nrhs == key->nrhs &&
-        c->bf == key->bf) {
-      *out = *c;
+  tune_load();
+  for (int i = 0; i < tune_cached && !hit; i++)
+    if (same_key(&tune_cache[i], key)) {
+      *out = tune_cache[i];
       hit = 1;
     }
-  }
   pthread_mutex_unlock(&tune_lock);
   return hit;
 }
 
 static void cache_put(const tuned_t *v) {
   pthread_mutex_lock(&tune_lock);
-  tune_cache[tune_cached < TUNE_CACHE ? tune_cached++ : TUNE_CACHE - 1] = *v;
+  cache_store(v);
+  tune_append(v);
   pthread_mutex_unlock(&tune_lock);
 }
 
@@ -508,8 +609,9 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
       shape_t c[40];
       int nc;
       float best = 0.0f;
-      if (cache_find(&key, &hit)) {
-        /* only the cached shape; timed once more for the report */
+      int cached = cache_find(&key, &hit);
+      if (cached) {
+        /* only the cached shape, with its recorded time (no timing run) */
         c[0] = hit.shape;
         nc = 1;
         best = 1e30f;
@@ -545,8 +647,9 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         tb->bf[op] = bf;
         tb->d_sched[op] = sched;
         tb->sched_len[op] = slen;
-        float ms;
-        rc = time_run(pl, op, kind, l, stream, &ms);
+        float ms = 0.0f;
+        /* a cached shape is taken as it is; candidates are timed */
+        if (!cached) rc = time_run(pl, op, kind, l, stream, &ms);
         if (!rc && ms < best) {
           best = ms;
           fmbh_free(psched);
@@ -561,16 +664,16 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
           fmbh_free(sched);
         }
       }
-      if (!rc && best >= 1e30f) rc = time_run(pl, op, kind, l, stream, &best);
+      if (!rc && cached) best = hit.ms;
       tb->tuned_ms[op] = best;
-      if (!rc) {
+      if (!rc && !cached) {
         key.shape.mode = tb->mode[op];
         key.shape.nt = tb->nt[op];
         key.shape.mt = tb->mt[op];
         key.shape.nw = tb->nw[op];
         key.shape.bf = tb->bf[op];
         key.ms = best;
-        if (!cache_find(&key, &hit)) cache_put(&key);
+        cache_put(&key);
       }
       free(coff);
     }
