@@ -76,7 +76,6 @@ _SIGS = {
     "fo_bem_fmm_setup": (_ci, [_vp, _ci, _ci, _vp]),
     "fo_bem_fmm_orders": (_ci, [_vp, _ip]),
     "fo_bem_fmm_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
-    "fo_bem_fmm_time_sample": (ctypes.c_double, [_vp, _ci, _dp]),
     "fo_bem_field": (None, [_vp, _ci, _dp, _dp, _dp, _ci, _vp, _dp, _dp]),
     "fo_bem_near_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_gmres": (_ci, [_vp, _ci, _ci, _cd, _ci, _dp, _dp, _dp, _dp, ctypes.POINTER(_ci),
@@ -375,13 +374,6 @@ class Bem:
         lib().fo_bem_field(self.h, nrhs, xr, xi, pts, pts.shape[0],
                            d.ctypes.data_as(ctypes.c_void_p) if d is not None else None, ore, oim)
         return ore + 1j * oim
-
-    def fmm_time_sample(self, stride):
-        """Estimated seconds of one FMM matvec (see fo_bem_fmm_time_sample)
-        and the measured phase seconds of the sample."""
-        ph = np.zeros(4)
-        est = lib().fo_bem_fmm_time_sample(self.h, int(stride), ph)
-        return est, ph[1:]
 
     def dense_row(self, i):
         re, im = np.zeros(self.n), np.zeros(self.n)

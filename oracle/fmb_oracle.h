@@ -137,12 +137,6 @@ int fo_bem_fmm_setup(fo_bem *b, int s, int depth, const int *plev);
 int fo_bem_fmm_orders(const fo_bem *b, int *p);
 void fo_bem_fmm_matvec(const fo_bem *b, int nrhs, const double *xre,
                        const double *xim, double *yre, double *yim);
-/* Estimated seconds of one FMM matvec from a timing sample: P2M and M2M in
- * full, M2L/L2L/L2P/near for the first 1/stride of the leaf boxes (and their
- * ancestors), each level's time scaled by its total / sampled boxes.
- * phases[1..3]: measured seconds of P2M+M2M, M2L+L2L, L2P+near on the
- * sample.  Benchmark use only. */
-double fo_bem_fmm_time_sample(const fo_bem *b, int stride, double *phases);
 void fo_bem_near_matvec(const fo_bem *b, int nrhs, const double *xre,
                         const double *xim, double *yre, double *yim);
 

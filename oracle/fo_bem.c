@@ -577,20 +577,8 @@ static const double *Hof(const fo_bem *b, int l, double cb) {
   return NULL;
 }
 
-/* One FMM matvec.  stride > 1 is a timing sample: M2L, L2L, L2P and the
- * near field run only for the first nbox/stride leaf boxes (Morton order)
- * and their ancestors, a contiguous prefix at every level that keeps at
- * least 64 boxes per thread (levels with fewer run whole); the other
- * outputs are left 0.  ph, when given, receives the seconds of P2M + M2M,
- * M2L + L2L and L2P + near (ph[1..3]). */
-static double now_s(void) {
-  struct timespec ts;
-  clock_gettime(CLOCK_MONOTONIC, &ts);
-  return ts.tv_sec + 1e-9 * ts.tv_nsec;
-}
-
-static void fmm_run(const fo_bem *b, const fo_cplx *x, fo_cplx *y, int stride,
-                    double *ph) {
+/* One FMM matvec: P2M, M2M, M2L + L2L, L2P + near field. */
+static void fmm_one(const fo_bem *b, const fo_cplx *x, fo_cplx *y) {
   const fo_tree *t = b->tree;
   int L = b->L, N = b->N;
   fo_cplx *M[16], *Lc[16];
@@ -600,7 +588,6 @@ static void fmm_run(const fo_bem *b, const fo_cplx *x, fo_cplx *y, int stride,
     Lc[l] = (fo_cplx *)calloc(n, sizeof(fo_cplx));
   }
   int pL = b->p[L], ps = pL * pL;
-  double t0 = now_s(), t1, t2, t3;
   /* P2M */
 #pragma omp parallel for schedule(dynamic, 8)
   for (int bx = 0; bx < t->nbox[L]; bx++) {
@@ -637,21 +624,10 @@ static void fmm_run(const fo_bem *b, const fo_cplx *x, fo_cplx *y, int stride,
     }
     free(cst);
   }
-  t1 = now_s();
-  double m2l_scaled = 0.0;
   /* M2L and L2L */
   for (int l = 2; l <= L; l++) {
     int pl = b->p[l], cs = fo_coax_size(pl);
     int nb = t->nbox[l];
-    double tl = now_s();
-    /* sample: the prefix of boxes that are ancestors of sampled leaves */
-    if (stride > 1) {
-      int cutL = t->nbox[L] / stride > 0 ? t->nbox[L] / stride : 1;
-      int kcut = t->keys[L][cutL - 1] >> (3 * (L - l)), c = 0;
-      while (c < nb && t->keys[l][c] <= kcut) c++;
-      /* small samples do not keep every thread busy: such levels run whole */
-      if (c >= 64 * omp_get_max_threads()) nb = c;
-    }
 #pragma omp parallel for schedule(dynamic, 4)
     for (int bx = 0; bx < nb; bx++) {
       fo_cplx *dst = Lc[l] + (size_t)bx * pl * pl;
@@ -676,17 +652,11 @@ static void fmm_run(const fo_bem *b, const fo_cplx *x, fo_cplx *y, int stride,
             Lc[l - 1] + (size_t)b->par[l][bx] * pp * pp, dst);
       }
     }
-    m2l_scaled += (now_s() - tl) * (double)t->nbox[l] / (nb > 0 ? nb : 1);
   }
-  t2 = now_s();
   /* L2P + near */
 #pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < N; i++) {
     int bx = t->elem_box[i];
-    if (stride > 1 && bx >= (t->nbox[L] / stride > 0 ? t->nbox[L] / stride : 1)) {
-      y[i] = 0.0;
-      continue;
-    }
     const fo_cplx *E = b->Eb + (size_t)i * ps;
     const fo_cplx *D = Lc[L] + (size_t)bx * ps;
     fo_cplx s = 0.0;
@@ -695,37 +665,10 @@ static void fmm_run(const fo_bem *b, const fo_cplx *x, fo_cplx *y, int stride,
       s += b->nr_val[q] * x[b->nr_col[q]];
     y[i] = s;
   }
-  t3 = now_s();
-  if (ph) {
-    int cutL = stride > 1 ? (t->nbox[L] / stride > 0 ? t->nbox[L] / stride : 1) : t->nbox[L];
-    ph[0] = m2l_scaled + (t3 - t2) * (double)t->nbox[L] / cutL; /* sampled, scaled */
-    ph[1] = t1 - t0; /* P2M + M2M */
-    ph[2] = t2 - t1;
-    ph[3] = t3 - t2;
-  }
   for (int l = 2; l <= L; l++) {
     free(M[l]);
     free(Lc[l]);
   }
-}
-
-static void fmm_one(const fo_bem *b, const fo_cplx *x, fo_cplx *y) {
-  fmm_run(b, x, y, 1, NULL);
-}
-
-double fo_bem_fmm_time_sample(const fo_bem *b, int stride, double *phases) {
-  int N = b->N;
-  fo_cplx *x = (fo_cplx *)malloc(sizeof(fo_cplx) * N);
-  fo_cplx *y = (fo_cplx *)malloc(sizeof(fo_cplx) * N);
-  for (int i = 0; i < N; i++) x[i] = cos(0.37 * i) + I * sin(0.11 * i);
-  double ph[4];
-  fmm_run(b, x, y, stride, ph);
-  free(x);
-  free(y);
-  if (phases)
-    for (int i = 0; i < 4; i++) phases[i] = ph[i];
-  /* sampled phases scaled level by level by total / sampled boxes */
-  return ph[1] + ph[0];
 }
 
 void fo_bem_fmm_matvec(const fo_bem *b, int nrhs, const double *xre,
