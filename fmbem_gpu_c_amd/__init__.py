@@ -100,6 +100,9 @@ _SIGS = {
     "fmb_gen_rot_mats": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _vp]),
     "fmb_gen_coax_mats": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int,
                                          ctypes.c_int, _vp]),
+    "fmb_gen_coax_mats_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _vp,
+                                                 ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "fmb_coax_scratch_bytes": (ctypes.c_size_t, []),
     "fmb_reduce_rot_mat": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int]),
     "fmb_reduce_coax_mat": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int]),
     "fmb_plan_params_default": (None, [ctypes.POINTER(PlanParams)]),
@@ -628,6 +631,24 @@ def _batch_rows(plans, nrhs, t, name):
     if shape != (tot, n) and not (tot == 1 and shape == (n,)):
         raise FmbError(-1, "%s: expected shape (%d, %d), got %s" % (name, tot, n, shape))
     return nrhs
+
+
+def gen_coax_mats_noalloc(kind, k, t, p, out=None):
+    """fmb_gen_coax_mats_noalloc: coaxial matrices of order p for the
+    distances t (a float64 CUDA tensor) into out (complex128 CUDA tensor
+    [len(t)][coax_size(p)], created when None) on the current stream, with
+    no device allocation inside the library."""
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.dim() == 1 and t.numel() >= 1):
+        raise FmbError(-1, "t: expected a 1-D tensor of distances")
+    n, cs = t.numel(), lib().fmb_coax_size(p)
+    if out is None:
+        out = torch.empty((n, cs), dtype=torch.complex128, device=t.device)
+    if tuple(out.shape) != (n, cs):
+        raise FmbError(-1, "out: expected shape (%d, %d), got %s" % (n, cs, tuple(out.shape)))
+    _check(lib().fmb_gen_coax_mats_noalloc(kind, float(k), _dev(t, torch.float64), n, p,
+                                           _dev(out), _stream()))
+    return out
 
 
 def reduce_rot_mat(mat, p1, p2):

@@ -55,6 +55,9 @@ int fmbh_sorted_geometry(fmb_tree *t, const double *d_V, const int *d_F,
 /* ---- operators (ops.hip) */
 int fmbh_gen_H(const double *h_beta, int nbeta, int P, double *d_H,
                void *stream);
+size_t fmbh_coax_scratch_bytes(void);
+int fmbh_gen_coax_static(int kind, double k, const double *d_t, int n, int P, fmb_c64 *d_out,
+                         void *stream);
 int fmbh_gen_coax(int kind, double k, const double *h_t, int n, int P,
                   fmb_c64 *d_out, void *stream);
 int fmbh_coax_scale(int prec, const fmb_c64 *d_in, int P, const double *d_rs,

@@ -14,6 +14,8 @@
 // Rotation: one lane per (m', m) column.  Coaxial: one workgroup per matrix
 // (sectorial pass) or per |m| block (advancing pass), sequential in m
 // (resp. n) with the orthogonal index spread over the workgroup.
+#include <mutex>
+
 #include "fmb_device.h"
 
 namespace {
@@ -302,5 +304,65 @@ extern "C" int fmbh_rot_phase(const double *d_H, int n, int P,
   FMB_LAUNCH_CHECK();
   FMB_HIP_CHECK(hipStreamSynchronize(st));
   fmbh_free(d_a);
+  return FMB_OK;
+}
+
+// ---- the coaxial generator without allocations ----
+// The sectorial table of fmbh_gen_coax (n * P * (2P - 1) complex values)
+// lives in a fixed device array instead of a fresh allocation, for callers
+// that cannot allocate.  Every call on a device shares that device's copy;
+// each call writes every entry it reads (k_coax_sect fills row m before
+// k_coax_adv reads it), so nothing needs clearing between calls, and calls
+// that overlap on one device race.
+#define FMB_COAX_SCRATCH (8u << 20) /* bytes */
+static __device__ __align__(256) double2 scratch[FMB_COAX_SCRATCH / sizeof(double2)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_lock;
+void *scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+// the device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address)
+static int scratch_address(void **out) {
+  int dev = 0;
+  FMB_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fmb_set_error(FMB_ERR_RANGE, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> g(scratch_lock);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fmb_set_error(FMB_ERR_HIP, "scratch address: %s", hipGetErrorString(e));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return FMB_OK;
+}
+
+extern "C" size_t fmbh_coax_scratch_bytes(void) { return FMB_COAX_SCRATCH; }
+
+extern "C" int fmbh_gen_coax_static(int kind, double k, const double *d_t, int n, int P,
+                                    fmb_c64 *d_out, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (P < 1 || 2 * P - 2 >= 512)
+    return fmb_set_error(FMB_ERR_RANGE, "coaxial order %d out of range", P);
+  const size_t need = sizeof(double2) * (size_t)n * P * (2 * P - 1);
+  if (need > FMB_COAX_SCRATCH)
+    return fmb_set_error(FMB_ERR_RANGE,
+                         "%d coaxial matrices of order %d need %zu bytes of scratch, "
+                         "above the %u available (n * p * (2p - 1) * 16 bytes)",
+                         n, P, need, FMB_COAX_SCRATCH);
+  void *S = nullptr;
+  int rc = scratch_address(&S);
+  if (rc) return rc;
+  int ld = 2 * P - 1;
+  hipLaunchKernelGGL(k_coax_sect, dim3(n), dim3(NT), 0, st, kind, k, d_t, P, (double2 *)S);
+  FMB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_coax_adv, dim3(P, n), dim3(NT), 3 * ld * sizeof(double2), st, P,
+                     (const double2 *)S, (double2 *)d_out);
+  FMB_LAUNCH_CHECK();
   return FMB_OK;
 }

@@ -49,6 +49,18 @@ int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
   return rc;
 }
 
+/* fmb_gen_coax_mats without allocations: device input and output, the
+ * recurrence table in the library's fixed device scratch (no counterpart in
+ * the reference). */
+int fmb_gen_coax_mats_noalloc(int kind, double k, const double *d_t, int n, int p,
+                              fmb_c64 *d_out, void *stream) {
+  if ((kind != FMB_COAX_RR && kind != FMB_COAX_SR) || !d_t || n < 1 || p < 1 || !d_out)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_gen_coax_mats_noalloc: invalid arguments");
+  return fmbh_gen_coax_static(kind, k, d_t, n, p, d_out, stream);
+}
+
+size_t fmb_coax_scratch_bytes(void) { return fmbh_coax_scratch_bytes(); }
+
 /* reduceSparseRotMat (translation.cu:22-26): the n-blocks n < p2 of a
  * rotation matrix of order p1 are its first fmb_rot_size(p2) entries. */
 int fmb_reduce_rot_mat(const fmb_c64 *in, int p1, fmb_c64 *out, int p2) {

@@ -16,6 +16,8 @@
 #ifndef FMBEM_H
 #define FMBEM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -116,6 +118,18 @@ int fmb_gen_rot_mats(const double *angles, int n, int p, fmb_c64 *out);
 #define FMB_COAX_SR 1
 int fmb_gen_coax_mats(int kind, double k, const double *t, int n, int p,
                       fmb_c64 *out);
+/* No counterpart in the reference: fmb_gen_coax_mats for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  d_t: n distances in device memory;
+ * d_out: n*fmb_coax_size(p) complex values in device memory, written on
+ * `stream` without a host synchronisation.  The recurrence table
+ * (n * p * (2p-1) * 16 bytes) lives in a fixed device scratch array of
+ * fmb_coax_scratch_bytes() bytes per device (8 MiB, held until the process
+ * ends); larger requests return FMB_ERR_RANGE.  No allocation is made.
+ * Calls that overlap on one device share the scratch and race. */
+int fmb_gen_coax_mats_noalloc(int kind, double k, const double *d_t, int n, int p,
+                              fmb_c64 *d_out, void *stream);
+size_t fmb_coax_scratch_bytes(void);
 /* reduceSparseRotMat (translation.cu:22-26; translation.h:132): a rotation
  * matrix of order p1 (host, reference n-block layout) truncated to order
  * p2 <= p1: its first fmb_rot_size(p2) values (in and out may alias). */
