@@ -162,7 +162,7 @@ def traffic_of(workload, lbig, shape):
 
 def plan_kwargs(a):
     return dict(autotune=0 if a.no_autotune else 1, serial_levels=1 if a.serial_levels else 0,
-                split_bf16=1 if a.split_bf16 else 0)
+                split_bf16=1 if a.split_bf16 else 0, ldso=1 if a.ldso else 0)
 
 
 def run_sphere(a, w, torch, F, log):
@@ -517,6 +517,8 @@ def main():
                     help="truncation eps of the refined solve's inner plan (0: no refined solve)")
     ap.add_argument("--inner-tol", type=float, default=1e-3,
                     help="relative tolerance of each inner solve of the refined solve")
+    ap.add_argument("--ldso", action="store_true",
+                    help="fixed tile-shape rule with the operators in LDS where they fit")
     ap.add_argument("--serial-levels", action="store_true",
                     help="all translations on one stream (per-level times without overlap)")
     ap.add_argument("--tune-cache", default=None,

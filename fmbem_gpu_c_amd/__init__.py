@@ -53,6 +53,7 @@ class PlanParams(ctypes.Structure):
         ("serial_levels", ctypes.c_int),
         ("waves", ctypes.c_int),
         ("split_bf16", ctypes.c_int),
+        ("ldso", ctypes.c_int),
     ]
 
 

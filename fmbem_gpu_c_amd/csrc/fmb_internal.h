@@ -91,6 +91,7 @@ typedef struct fmb_rcr_tab {
   void *d_Cpad[2];      /* per opset: zero-padded coaxial blocks           */
   float tuned_ms[2];    /* per opset: autotuned time of one launch (0: not tuned) */
   size_t cpad_stride;   /* complex values per coaxial matrix in d_Cpad     */
+  int hlen;             /* values of the padded rotation blocks n < P      */
 } fmb_rcr_tab;
 
 struct fmb_plan {

@@ -87,6 +87,7 @@ int fmbh_m2l(const fmb_plan *pl, int op, int l, int nrhs, void *stream);
 #define FMBH_RCR_LDS 1 /* batch in LDS, blocks owned by one wave          */
 #define FMBH_RCR_GWS 2 /* batch in a global workspace, row-group units    */
 #define FMBH_RCR_LDSP 3 /* as LDS, a persistent grid looping over batches */
+#define FMBH_RCR_LDSO 4 /* as LDSP, with the bucket's operators in LDS      */
 typedef struct fmbh_rcr_job {
   int mode, nt, mt, nw; /* nw: waves per workgroup, 4 or 8 */
   int bf;               /* split-BF16 rotations (FP32 pass, LDS modes)   */
@@ -105,6 +106,7 @@ typedef struct fmbh_rcr_job {
   int P, pin, pout;
   const int *sched;
   int sched_len;
+  int hlen;          /* values of the padded rotation blocks n < P        */
   const void *in;
   void *out;
   void *gws;
@@ -117,6 +119,8 @@ int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt);
 size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len);
 int fmbh_rcr_mt_lds(int pm);
 size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt);
+/* LDSO-mode bytes of a workgroup (0: no LDS-mode unit shape for pm) */
+size_t fmbh_rcr_ldso_bytes(int pm, int prec, int nt, int sched_len, int hlen, size_t clen);
 int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
 /* zero-padded operator copies: rotation block n at hoff[n] (16-row x
  * 4-column multiples, leading dimension = padded rows); coaxial block |m|

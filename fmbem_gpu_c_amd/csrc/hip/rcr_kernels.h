@@ -34,6 +34,12 @@
 //   GWS: X in a per-workgroup global workspace, two buffers ping-ponged
 //        between stages, so blocks split into row groups of 16*MT rows that
 //        any wave may take; a persistent grid walks the batches.
+//   LDSO: as LDS, on a persistent grid whose workgroups each take a
+//        contiguous range of batches (so of one or two buckets) and hold the
+//        bucket's padded H and C blocks in LDS, copied in when the bucket's
+//        polar angle or distance changes: operand A is then read from LDS
+//        instead of L2 (low orders only, where operators and batch fit the
+//        160 KB).
 // A workgroup has W = 4 or 8 waves (a tile-shape choice).  Targets receive
 // hardware float atomic adds; the lanes of a wave add consecutive reals of
 // one target vector, so each instruction fills whole cache lines.
@@ -106,6 +112,7 @@ struct RcrArgs {
   int P, pin, pout;
   const int *sched;    // offsets [3][W + 1], then {code, operator offset}
   int sched_len;
+  int hlen;            // LDSO: T values of the padded H blocks n < P
   const void *Min;     // [box][nrhs][pin^2]
   void *Lout;          // [box][nrhs][pout^2], added to
   int nrhs;
@@ -461,18 +468,27 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
 }
 
 // One batch: vectors u0 .. u0+V-1 of chunk ch.
-template <typename T, int NT, int MT, int W, bool GWS, bool BF = false>
+// OPL: the bucket's operators are Hs / Cs (LDS copies) instead of global
+template <typename T, int NT, int MT, int W, bool GWS, bool BF = false, bool OPL = false>
 __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
                                           T *X0, T *X1, cplx<T> *ph,
-                                          const int *sched) {
+                                          const int *sched, const T *Hs = nullptr,
+                                          const cplx<T> *Cs = nullptr) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
   const int tid = threadIdx.x, wave = uni(tid >> 6);
   const int pin = a.pin, pout = a.pout;
   const int Pm = pin > pout ? pin : pout;
   const int Pi2 = pin * pin, Po2 = pout * pout, Pm2 = Pm * Pm;
   const OffInfo oi = a.info[ch.x];
-  const T *H = (const T *)a.H + a.hstride * oi.beta;
-  const cplx<T> *C = (const cplx<T> *)a.C + a.cstride * oi.dist;
+  const T *H;
+  const cplx<T> *C;
+  if constexpr (OPL) {
+    H = Hs;
+    C = Cs;
+  } else {
+    H = (const T *)a.H + a.hstride * oi.beta;
+    C = (const cplx<T> *)a.C + a.cstride * oi.dist;
+  }
   const int U = (ch.z - ch.y) * a.nrhs;
   __syncthreads();
   const cplx<T> *phb = (const cplx<T> *)a.phase + (size_t)a.pstride * ch.x;
@@ -590,6 +606,56 @@ __global__ void __launch_bounds__(kThreads<W>)
   }
 }
 
+// LDSO mode: persistent grid; workgroup g takes items [g I / G, (g+1) I / G)
+// of the I = nchunks * nb items (chunk-major, so the range covers one or two
+// buckets) and keeps the current bucket's padded rotation blocks (n < P) and
+// coaxial blocks in LDS.  Two waves per SIMD at most: the LDS holds one
+// workgroup per CU.
+template <typename T, int W> constexpr int kWavesPerEuO = W == 8 ? 2 : 1;
+
+__device__ __forceinline__ size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+template <typename T, int NT, int MT, int W>
+__global__ void __launch_bounds__(kThreads<W>)
+    __attribute__((amdgpu_waves_per_eu(kWavesPerEuO<T, W>))) k_rcr_ldso(RcrArgs a) {
+  constexpr int V = 8 * NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int Pm = a.pin > a.pout ? a.pin : a.pout;
+  T *X = (T *)smem;
+  cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * Row<NT>::stride);
+  int *sched = (int *)(ph + Pm + (Pm & 1));
+  T *Hs = (T *)(smem + align16((size_t)((unsigned char *)(sched + a.sched_len) - smem)));
+  cplx<T> *Cs = (cplx<T> *)(Hs + a.hlen + (a.hlen & 1));
+  for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
+  const long long items = (long long)a.nchunks * a.nb;
+  const long long i0 = blockIdx.x * items / gridDim.x;
+  const long long i1 = (blockIdx.x + 1) * items / gridDim.x;
+  int beta = -1, dist = -1;
+  const int clen = (int)a.cstride;
+  for (long long it = i0; it < i1; it++) {
+    const int c = (int)(it / a.nb), b = (int)(it % a.nb);
+    const int4 ch = a.chunks[c];
+    if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
+    const OffInfo oi = a.info[ch.x];
+    if (oi.beta != beta || oi.dist != dist) {
+      // every wave is past the previous batch's last operator read
+      __syncthreads();
+      if (oi.beta != beta) {
+        const T *src = (const T *)a.H + a.hstride * oi.beta;
+        for (int i = threadIdx.x; i < a.hlen; i += kThreads<W>) Hs[i] = src[i];
+      }
+      if (oi.dist != dist) {
+        const cplx<T> *src = (const cplx<T> *)a.C + a.cstride * oi.dist;
+        for (int i = threadIdx.x; i < clen; i += kThreads<W>) Cs[i] = src[i];
+      }
+      beta = oi.beta;
+      dist = oi.dist;
+      // rcr_batch opens with a barrier, which publishes the copies
+    }
+    rcr_batch<T, NT, MT, W, false, false, true>(a, ch, b * V, X, X, ph, sched, Hs, Cs);
+  }
+}
+
 // GWS mode: persistent grid over items chunk * nb + batch.
 template <typename T, int NT, int MT, int W>
 __global__ void __launch_bounds__(kThreads<W>) k_rcr_gws(RcrArgs a) {
@@ -625,8 +691,16 @@ inline size_t small_lds(int pm, size_t tsz, int sched_len) {
   return (size_t)(pm + (pm & 1)) * 2 * tsz + sizeof(int) * (size_t)sched_len;
 }
 
+
 inline size_t x_bytes(int pm, int nt, size_t tsz) {
   return (size_t)pm * pm * (16 * nt + 2) * tsz;
+}
+
+// LDSO mode: batch, phases, schedule, then (16-byte aligned) hlen rotation
+// values and clen complex coaxial values
+inline size_t ldso_bytes(int pm, int nt, size_t tsz, int sched_len, int hlen, size_t clen) {
+  size_t b = (x_bytes(pm, nt, tsz) + small_lds(pm, tsz, sched_len) + 15) & ~(size_t)15;
+  return b + (size_t)(hlen + (hlen & 1)) * tsz + clen * 2 * tsz;
 }
 
 // Raises the kernel's dynamic-LDS cap to the CU's 160 KB (not to this
@@ -669,6 +743,7 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   a.pout = j->pout;
   a.sched = j->sched;
   a.sched_len = j->sched_len;
+  a.hlen = j->hlen;
   a.Min = j->in;
   a.Lout = j->out;
   a.nrhs = nrhs;
@@ -693,6 +768,24 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
     long long items = (long long)a.nchunks * a.nb, cap = (long long)per_cu * ncu;
     int grid = (int)(items < cap ? items : cap);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads<W>), lds, st, a);
+  } else if (j->mode == FMBH_RCR_LDSO) {
+    if constexpr (BF) {
+      return fmb_set_error(FMB_ERR_RANGE, "split-BF16 rotations have no LDSO tile shape");
+    } else {
+      const size_t lds = ldso_bytes(Pm, NT, tsz, j->sched_len, j->hlen, j->cstride);
+      auto kern = k_rcr_ldso<T, NT, MT, W>;
+      int rc = set_lds(kern, lds);
+      if (rc) return rc;
+      int per_cu = 0, ncu = 0, dev = 0;
+      FMB_HIP_CHECK(
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads<W>, lds));
+      FMB_HIP_CHECK(hipGetDevice(&dev));
+      FMB_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      if (per_cu < 1) per_cu = 1;
+      long long items = (long long)a.nchunks * a.nb, cap = (long long)per_cu * ncu;
+      int grid = (int)(items < cap ? items : cap);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads<W>), lds, st, a);
+    }
   } else if constexpr (BF) {
     return fmb_set_error(FMB_ERR_RANGE, "split-BF16 translations need an LDS tile shape");
   } else {

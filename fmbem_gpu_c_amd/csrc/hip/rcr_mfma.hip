@@ -82,6 +82,13 @@ extern "C" size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len) {
 
 extern "C" int fmbh_rcr_mt_lds(int pm) { return mt_lds(pm); }
 
+extern "C" size_t fmbh_rcr_ldso_bytes(int pm, int prec, int nt, int sched_len, int hlen,
+                                      size_t clen) {
+  const size_t tsz = prec == FMB_FP64 ? sizeof(double) : sizeof(float);
+  if (!mt_lds(pm)) return 0;
+  return ldso_bytes(pm, nt, tsz, sched_len, hlen, clen);
+}
+
 extern "C" size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt) {
   const size_t tsz = prec == FMB_FP64 ? sizeof(double) : sizeof(float);
   return 2 * x_bytes(pm, nt, tsz);

@@ -99,7 +99,7 @@ static void rot_offsets_bf(int np, int *hb) {
 /* whether opset op may run mode's rotations in split-BF16 */
 static int use_bf(const fmb_plan *pl, int op, int mode) {
   return pl->prm.split_bf16 && pl->ops[op].prec != FMB_FP64 &&
-         (mode == FMBH_RCR_LDS || mode == FMBH_RCR_LDSP);
+         (mode == FMBH_RCR_LDS || mode == FMBH_RCR_LDSP);  /* not LDSO */
 }
 
 /* offsets of the padded coaxial blocks |m| < min(pin, pout) (+1 values) */
@@ -247,6 +247,7 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
   if (!coff) return fmb_set_error(FMB_ERR_NOMEM, "coaxial offsets");
   coax_offsets(pin, pout, coff);
   tb->cpad_stride = (size_t)coff[mn];
+  tb->hlen = hoff[tb->P];
   for (int op = 0; op < pl->nops && !rc; op++) {
     fmb_opset *o = &pl->ops[op];
     tb->mode[op] = 0;
@@ -254,6 +255,16 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
     if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
       break;
     tb->nw[op] = pl->prm.waves ? pl->prm.waves : 4;
+    /* operators in LDS where batch and operators fit (the schedule bound of
+     * fmbh_rcr_shape) */
+    if (pl->prm.ldso && tb->mode[op] != FMBH_RCR_GWS)
+      for (int nt = 2; nt >= 1; nt--)
+        if (fmbh_rcr_ldso_bytes(tb->P, o->prec, nt, 8 * tb->P + 16, tb->hlen,
+                                tb->cpad_stride) <= 160 * 1024) {
+          tb->mode[op] = FMBH_RCR_LDSO;
+          tb->nt[op] = nt;
+          break;
+        }
     tb->bf[op] = use_bf(pl, op, tb->mode[op]);
     if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op],
                           tb->bf[op] ? hboff : hoff, coff, &tb->d_sched[op],
@@ -415,6 +426,11 @@ static int shape_candidates(int P, int bfok, shape_t *c) {
           c[n++] = w;
         }
     for (int nt = 1; nt <= 2; nt++)
+      if (mt) {
+        shape_t v = {FMBH_RCR_LDSO, nt, mt, nw, 0};
+        c[n++] = v;
+      }
+    for (int nt = 1; nt <= 2; nt++)
       for (int i = 0; i < 4; i++) {
         shape_t v = {FMBH_RCR_GWS, nt, mts[i], nw, 0};
         c[n++] = v;
@@ -454,7 +470,7 @@ typedef struct {
   float ms;
 } tuned_t;
 #define TUNE_CACHE 4096
-#define TUNE_MAGIC "fmbem-rcr-tile-v1"
+#define TUNE_MAGIC "fmbem-rcr-tile-v2"
 static tuned_t tune_cache[TUNE_CACHE];
 static int tune_cached, tune_loaded;
 static char tune_dev[64];
@@ -527,10 +543,11 @@ static void tune_load(void) {
       continue;
     if (strcmp(magic, TUNE_MAGIC) || strcmp(dev, tune_dev)) continue;
     /* only shapes this build can launch; split-BF16 only where allowed */
-    if (v.shape.mode < FMBH_RCR_LDS || v.shape.mode > FMBH_RCR_LDSP || v.shape.nt < 1 ||
+    if (v.shape.mode < FMBH_RCR_LDS || v.shape.mode > FMBH_RCR_LDSO || v.shape.nt < 1 ||
         v.shape.nt > 2 || (v.shape.nw != 4 && v.shape.nw != 8) || v.kind < 0 || v.kind > 2 ||
         (v.shape.mt != 2 && v.shape.mt != 3 && v.shape.mt != 4 && v.shape.mt != 6) ||
-        (v.shape.bf && (!v.bf || v.shape.mode == FMBH_RCR_GWS)) || !(v.ms >= 0))
+        (v.shape.bf && (!v.bf || v.shape.mode == FMBH_RCR_GWS ||
+                        v.shape.mode == FMBH_RCR_LDSO)) || !(v.ms >= 0))
       continue;
     cache_store(&v);
   }
@@ -606,7 +623,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
       coax_offsets(tb->pin, tb->pout, coff);
       tuned_t key = {o->prec, kind, tb->pin, tb->pout, pairs->nchunks, pl->nrhs_max,
                      use_bf(pl, op, FMBH_RCR_LDS), {0, 0, 0, 0, 0}, 0.0f}, hit;
-      shape_t c[40];
+      shape_t c[48];
       int nc;
       float best = 0.0f;
       int cached = cache_find(&key, &hit);
@@ -628,8 +645,12 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, c[i].nw,
                               bf ? hboff : hoff, coff, &sched, &slen, stream)))
           break;
-        if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP) {
-          size_t b = fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
+        if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP ||
+            c[i].mode == FMBH_RCR_LDSO) {
+          size_t b = c[i].mode == FMBH_RCR_LDSO
+                         ? fmbh_rcr_ldso_bytes(tb->P, o->prec, c[i].nt, slen, tb->hlen,
+                                               tb->cpad_stride)
+                         : fmbh_rcr_lds_bytes(tb->P, o->prec, c[i].nt, slen);
           if (!b || b > 160 * 1024) {
             fmbh_free(sched);
             continue;
@@ -786,6 +807,7 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.pout = tb->pout;
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
+  j.hlen = tb->hlen;
   j.gws = (pl->aux && stream == pl->aux) ? (char *)o->d_gws + o->gws_slot : o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase[kind];
@@ -854,6 +876,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.pout = tb->pout;
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
+  j.hlen = tb->hlen;
   j.gws = o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase_c[kind];

@@ -192,6 +192,12 @@ typedef struct fmb_plan_params {
                                 split-BF16 (hi + lo bf16 operands, three
                                 products, FP32 accumulation; about 2^-16
                                 relative per product)                       */
+  int ldso;                  /* 1: in the fixed tile-shape rule, every
+                                translation whose batch and operator blocks
+                                fit in LDS together (FP32 orders up to about
+                                20) keeps the operators there on a
+                                persistent grid; autotune tries that shape
+                                either way                                  */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);

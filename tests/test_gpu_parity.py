@@ -410,3 +410,61 @@ def test_split_bf16_retune_and_graph():
         err = np.linalg.norm(y - want) / np.linalg.norm(want)
         print("graph replay", rep, "rel err %.2e" % err, flush=True)
         assert err < 1e-6
+
+
+@pytest.mark.parametrize("freq", [4000.0, 12000.0])
+def test_operators_in_lds_match(freq):
+    """Translations with the bucket's operator blocks held in LDS on a
+    persistent grid (ldso = 1: FP32 orders up to about 20, FP64 up to 16)
+    against the default tile shapes on the 20,480-triangle sphere at depth
+    6, forward and transposed passes: FP64 1e-13, FP32 1e-6 relative (the
+    same arithmetic, summation order of the target atomics only)."""
+    m = F.Mesh.icosphere(5, 0.1)
+    t = F.Tree(m, depth=6)
+    k = wavenumber(freq)
+    kw = dict(precision=F.FMB_MIXED, nrhs_max=3)
+    ref = F.Plan(t, k, **kw)
+    opl = F.Plan(t, k, ldso=1, **kw)
+    _, L, _, _ = t.info()
+    modes = {prec: [opl.tile_shape(kind, l, prec)[0] for kind in range(3)
+                    for l in range(2 if kind == 0 else 3, L + 1)]
+             for prec in (F.FMB_FP64, F.FMB_MIXED)}
+    print("f", freq, "orders", opl.orders()[2:L + 1].tolist(), "modes", modes, flush=True)
+    assert 4 in modes[F.FMB_MIXED]
+    n = m.arrays()[1].shape[0]
+    rng = np.random.default_rng(67)
+    x = torch.tensor(rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n)), device="cuda")
+    for prec, tol in ((F.FMB_FP64, 1e-13), (F.FMB_MIXED, 1e-6)):
+        for rep in range(2):
+            a = ref.matvec(x, prec=prec).cpu().numpy()
+            b = opl.matvec(x, prec=prec).cpu().numpy()
+            e = np.linalg.norm(a - b) / np.linalg.norm(a)
+            assert e < tol, (prec, rep, e)
+        at = ref.matvec_t(x, prec=prec).cpu().numpy()
+        bt = opl.matvec_t(x, prec=prec).cpu().numpy()
+        assert np.linalg.norm(at - bt) / np.linalg.norm(at) < tol
+
+
+def test_operators_in_lds_match_oracle():
+    """The LDS-operator translations against the oracle FMM: the
+    5,120-triangle sphere at depth 5, 4 kHz, orders [20, 18, 16, 14] so that
+    every level runs that mode in FP32 (and levels up to order 16 in FP64);
+    FP64 pass 1e-9, FP32 pass 1e-5."""
+    m = F.Mesh.icosphere(4, 0.1)
+    V, Fc = m.arrays()
+    t = F.Tree(m, depth=5)
+    k = wavenumber(4000.0)
+    orders = [20, 18, 16, 14]
+    pl = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=2, ldso=1, p_rule=F.PRULE_EXPLICIT,
+                p_levels=[0, 0] + orders)
+    assert all(pl.tile_shape(0, l, F.FMB_MIXED)[0] == 4 for l in range(2, 6))
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(depth=5, plev=dict(zip(range(2, 6), orders)))
+    rng = np.random.default_rng(71)
+    x = rng.standard_normal((2, Fc.shape[0])) + 1j * rng.standard_normal((2, Fc.shape[0]))
+    want = ob.fmm_matvec(x)
+    xt = torch.tensor(x, device="cuda")
+    e64 = np.linalg.norm(pl.matvec(xt, prec=F.FMB_FP64).cpu().numpy() - want) / np.linalg.norm(want)
+    e32 = np.linalg.norm(pl.matvec(xt, prec=F.FMB_MIXED).cpu().numpy() - want) / np.linalg.norm(want)
+    print("LDS operators vs oracle: FP64 %.2e, FP32 %.2e" % (e64, e32), flush=True)
+    assert e64 < 1e-9 and e32 < 1e-5
