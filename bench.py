@@ -162,7 +162,8 @@ def traffic_of(workload, lbig, shape):
 
 def plan_kwargs(a):
     return dict(autotune=0 if a.no_autotune else 1, serial_levels=1 if a.serial_levels else 0,
-                split_bf16=1 if a.split_bf16 else 0, ldso=1 if a.ldso else 0)
+                split_bf16=1 if a.split_bf16 else 0, ldso=1 if a.ldso else 0,
+                m2l_tile=a.m2l_tile)
 
 
 def run_sphere(a, w, torch, F, log):
@@ -519,6 +520,9 @@ def main():
                     help="relative tolerance of each inner solve of the refined solve")
     ap.add_argument("--ldso", action="store_true",
                     help="fixed tile-shape rule with the operators in LDS where they fit")
+    ap.add_argument("--m2l-tile", type=int, default=256,
+                    help="M2L work in tile-major order over this many target boxes (0: "
+                         "offset-major)")
     ap.add_argument("--serial-levels", action="store_true",
                     help="all translations on one stream (per-level times without overlap)")
     ap.add_argument("--tune-cache", default=None,

@@ -54,6 +54,7 @@ class PlanParams(ctypes.Structure):
         ("waves", ctypes.c_int),
         ("split_bf16", ctypes.c_int),
         ("ldso", ctypes.c_int),
+        ("m2l_tile", ctypes.c_int),
     ]
 
 

@@ -107,6 +107,8 @@ typedef struct fmbh_rcr_job {
   const int *sched;
   int sched_len;
   int hlen;          /* values of the padded rotation blocks n < P        */
+  int xcd;           /* 1: LDS mode on a 1-D grid, each XCD taking a
+                        contiguous range of the (chunk, batch) items      */
   const void *in;
   void *out;
   void *gws;

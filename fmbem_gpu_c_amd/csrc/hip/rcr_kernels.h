@@ -113,6 +113,7 @@ struct RcrArgs {
   const int *sched;    // offsets [3][W + 1], then {code, operator offset}
   int sched_len;
   int hlen;            // LDSO: T values of the padded H blocks n < P
+  int xcd;             // LDS: 1-D grid, XCD x takes items [x I / 8, (x+1) I / 8)
   const void *Min;     // [box][nrhs][pin^2]
   void *Lout;          // [box][nrhs][pout^2], added to
   int nrhs;
@@ -574,8 +575,20 @@ __global__ void __launch_bounds__(kThreads<W>)
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
   extern __shared__ __align__(16) unsigned char smem[];
   const int Pm = a.pin > a.pout ? a.pin : a.pout;
-  const int4 ch = a.chunks[blockIdx.x];
-  const int u0 = blockIdx.y * V;
+  int cx = blockIdx.x, by = blockIdx.y;
+  if (a.xcd) {
+    // workgroups are dealt to the 8 XCDs round-robin: workgroup g runs on
+    // XCD g % 8, which takes the contiguous item range x * per .. x * per +
+    // per - 1, so that neighbouring chunks share the XCD's L2
+    const long long items = (long long)a.nchunks * a.nb;
+    const long long per = (items + 7) / 8;
+    const long long it = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (it >= items || (blockIdx.x >> 3) >= per) return;
+    cx = (int)(it / a.nb);
+    by = (int)(it % a.nb);
+  }
+  const int4 ch = a.chunks[cx];
+  const int u0 = by * V;
   if (u0 >= (ch.z - ch.y) * a.nrhs) return;
   T *X = (T *)smem;
   cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * RS);
@@ -744,6 +757,7 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   a.sched = j->sched;
   a.sched_len = j->sched_len;
   a.hlen = j->hlen;
+  a.xcd = j->xcd;
   a.Min = j->in;
   a.Lout = j->out;
   a.nrhs = nrhs;
@@ -754,7 +768,13 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
     auto kern = k_rcr_lds<T, NT, MT, W, BF>;
     int rc = set_lds(kern, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(kern, dim3(a.nchunks, a.nb), dim3(kThreads<W>), lds, st, a);
+    if (a.xcd) {
+      const long long items = (long long)a.nchunks * a.nb, grid = 8 * ((items + 7) / 8);
+      if (grid > 0x7fffffffLL) return fmb_set_error(FMB_ERR_RANGE, "translation grid too large");
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads<W>), lds, st, a);
+    } else {
+      hipLaunchKernelGGL(kern, dim3(a.nchunks, a.nb), dim3(kThreads<W>), lds, st, a);
+    }
   } else if (j->mode == FMBH_RCR_LDSP) {
     const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_ldsp<T, NT, MT, W, BF>;

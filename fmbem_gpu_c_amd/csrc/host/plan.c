@@ -38,6 +38,7 @@ void fmb_plan_params_default(fmb_plan_params *p) {
   p->near_ratio = 2.0;
   p->nedge = 24;
   p->nrhs_max = 1;
+  p->m2l_tile = 256; /* measured: level-7 M2L 70.0 -> 66.5 ms (FP32), 128 -> 116 ms (FP64) */
 }
 
 static const int k_dist2[FMB_NDIST] = {4, 5, 6, 8, 9, 10, 11, 12,

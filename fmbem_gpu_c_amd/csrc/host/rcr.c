@@ -201,6 +201,62 @@ static int build_chunks(const int *start, int nbucket, fmb_rcr_tab *tb,
   return rc;
 }
 
+/* M2L chunks in tile-major order: pairs of target boxes [tile * T, (tile+1)
+ * * T) for every offset, then the next tile; within a (tile, offset) group
+ * chunks of at most RCR_CHUNK pairs.  start: the offset buckets of the
+ * tree's offset-major pair list, whose targets are in box order inside each
+ * bucket. */
+static int build_chunks_tiled(const fmb_tree *t, int l, int T, fmb_rcr_tab *tb, void *stream) {
+  const int *start = t->h_off_start[l];
+  int np = start[FMB_NOFF] - start[0], nbox = t->nbox[l], rc;
+  int ntile = (nbox + T - 1) / T;
+  int *tg = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+  /* per (tile, bucket): chunks counted, then placed */
+  int *cnt = (int *)calloc((size_t)ntile * FMB_NOFF + 1, sizeof(int));
+  int *c = (int *)malloc(sizeof(int) * 4 * (np / RCR_CHUNK + (size_t)ntile * FMB_NOFF + 1));
+  if (!tg || !cnt || !c) {
+    free(tg);
+    free(cnt);
+    free(c);
+    return fmb_set_error(FMB_ERR_NOMEM, "translation chunks");
+  }
+  rc = fmbh_d2h(tg, t->d_pair_tgt[l], sizeof(int) * np, stream);
+  int n = 0;
+  if (!rc) {
+    /* group boundaries inside each bucket: runs of one tile */
+    for (int b = 0; b < FMB_NOFF; b++)
+      for (int s0 = start[b]; s0 < start[b + 1];) {
+        int tile = tg[s0] / T, e = s0;
+        while (e < start[b + 1] && tg[e] / T == tile) e++;
+        cnt[(size_t)tile * FMB_NOFF + b + 1] += (e - s0 + RCR_CHUNK - 1) / RCR_CHUNK;
+        s0 = e;
+      }
+    for (size_t i = 1; i <= (size_t)ntile * FMB_NOFF; i++) cnt[i] += cnt[i - 1];
+    for (int b = 0; b < FMB_NOFF; b++)
+      for (int s0 = start[b]; s0 < start[b + 1];) {
+        int tile = tg[s0] / T, e = s0;
+        while (e < start[b + 1] && tg[e] / T == tile) e++;
+        int q = cnt[(size_t)tile * FMB_NOFF + b];
+        for (int f = s0; f < e; f += RCR_CHUNK, q++) {
+          c[4 * q] = b;
+          c[4 * q + 1] = f;
+          c[4 * q + 2] = f + RCR_CHUNK < e ? f + RCR_CHUNK : e;
+          c[4 * q + 3] = 0;
+        }
+        cnt[(size_t)tile * FMB_NOFF + b] = q;
+        s0 = e;
+      }
+    n = cnt[(size_t)ntile * FMB_NOFF];
+  }
+  tb->nchunks = n;
+  if (!rc) rc = fmbh_malloc(&tb->d_chunks, sizeof(int) * 4 * (n > 0 ? n : 1));
+  if (!rc && n) rc = fmbh_h2d(tb->d_chunks, c, sizeof(int) * 4 * n, stream);
+  free(tg);
+  free(cnt);
+  free(c);
+  return rc;
+}
+
 /* (parent, child) pairs of level l bucketed by the child's octant */
 static int octant_pairs(const fmb_tree *t, int l, fmb_rcr_tab *tb, void *stream) {
   int nb = t->nbox[l], rc;
@@ -731,7 +787,11 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     for (int l = 2; l <= L && !rc; l++) {
       fmb_rcr_tab *tb = &pl->rcr[0][l];
       void *C[2] = {pl->ops[0].d_Csr[l], pl->ops[1].d_Csr[l]};
-      if ((rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream))) break;
+      if (pl->prm.m2l_tile > 0)
+        rc = build_chunks_tiled(t, l, pl->prm.m2l_tile, tb, stream);
+      else
+        rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream);
+      if (rc) break;
       rc = plan_tab(pl, tb, pl->p[l], pl->p[l], C, FMB_NDIST, hoff, hboff, stream);
     }
     for (int l = 3; l <= L && !rc; l++) {
@@ -808,6 +868,7 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
   j.hlen = tb->hlen;
+  j.xcd = kind == 0 && pl->prm.m2l_tile > 0 && j.mode == FMBH_RCR_LDS;
   j.gws = (pl->aux && stream == pl->aux) ? (char *)o->d_gws + o->gws_slot : o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase[kind];
@@ -877,6 +938,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
   j.hlen = tb->hlen;
+  j.xcd = kind == 0 && pl->prm.m2l_tile > 0 && j.mode == FMBH_RCR_LDS;
   j.gws = o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase_c[kind];

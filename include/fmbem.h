@@ -198,6 +198,13 @@ typedef struct fmb_plan_params {
                                 20) keeps the operators there on a
                                 persistent grid; autotune tries that shape
                                 either way                                  */
+  int m2l_tile;              /* 0: M2L work in offset-major order (all pairs
+                                of one offset, then the next); t > 0: tile-
+                                major order (the pairs of t consecutive
+                                target boxes for every offset, then the next
+                                tile), each XCD taking a contiguous range of
+                                tiles, so that a tile's sources and targets
+                                stay in its caches (default 256)            */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);
