@@ -55,6 +55,7 @@ class PlanParams(ctypes.Structure):
         ("split_bf16", ctypes.c_int),
         ("ldso", ctypes.c_int),
         ("m2l_tile", ctypes.c_int),
+        ("m2l_tile_min_mb", ctypes.c_int),
     ]
 
 

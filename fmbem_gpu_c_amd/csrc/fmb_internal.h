@@ -92,6 +92,7 @@ typedef struct fmb_rcr_tab {
   float tuned_ms[2];    /* per opset: autotuned time of one launch (0: not tuned) */
   size_t cpad_stride;   /* complex values per coaxial matrix in d_Cpad     */
   int hlen;             /* values of the padded rotation blocks n < P      */
+  int tiled;            /* M2L: chunks in tile-major order (else offset-major) */
 } fmb_rcr_tab;
 
 struct fmb_plan {

@@ -64,11 +64,13 @@ __device__ __forceinline__ void atomic_add_c(cplx<T> *dst, cplx<T> v) {
   unsafeAtomicAdd(&dst->im, v.im);
 }
 
-/* n with n^2 <= f < (n+1)^2: the degree of coefficient index f = n^2+n+m */
+/* n with n^2 <= f < (n+1)^2: the degree of coefficient index f = n^2+n+m.
+ * For 0 <= f < 2^22 the float square root is within one of n, so one
+ * correction each way makes it exact (no data-dependent loop). */
 __device__ __forceinline__ int isqrt_fix(int f) {
   int n = (int)sqrtf((float)f);
-  while (n * n > f) n--;
-  while ((n + 1) * (n + 1) <= f) n++;
+  n -= n * n > f;
+  n += (n + 1) * (n + 1) <= f;
   return n;
 }
 

@@ -119,21 +119,24 @@ typedef struct fmbh_rcr_job {
 int fmbh_rcr_shape(int pm, int prec, int *mode, int *nt, int *mt);
 /* global workspace bytes of one workgroup in GWS mode */
 size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len);
-int fmbh_rcr_mt_lds(int pm);
+int fmbh_rcr_mt_lds(int pm, int bf);
 size_t fmbh_rcr_gws_bytes(int pm, int prec, int nt);
 /* LDSO-mode bytes of a workgroup (0: no LDS-mode unit shape for pm) */
 size_t fmbh_rcr_ldso_bytes(int pm, int prec, int nt, int sched_len, int hlen, size_t clen);
 int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream);
-/* zero-padded operator copies: rotation block n at hoff[n] (16-row x
- * 4-column multiples, leading dimension = padded rows); coaxial block |m|
- * (rows pout-|m|, columns pin-|m|) at coff[|m|] */
-int fmbh_pad_rot(int prec, const void *d_H, int nbeta, int Pmax,
+/* zero-padded operator copies: degree n's even/odd rotation blocks (from
+ * the FP64 blocks, rounded to the opset's type) at hoff[n], the even block
+ * ((n+1) x (n+1), index i for m = i + 1, i < n, and m = 0) then the odd one
+ * (n x n, index i for |m| = i + 1), each to 16-row x 4-column multiples,
+ * leading dimension = padded rows; coaxial block |m| (rows pout-|m|,
+ * columns pin-|m|) at coff[|m|] */
+int fmbh_pad_rot(int prec, const double *d_H64, int nbeta, int Pmax,
                  const int *d_hoff, size_t pstride, void *d_Hp, void *stream);
-/* split-BF16 copies of the FP32 padded rotation blocks: block n of beta b
- * at hboff[n] (rows padded to 16, K to 32, row-major), hi part at
- * b * 2 half, lo part half values further */
-int fmbh_split_rot(const void *d_Hp, size_t pstride, const int *d_hoff, int nbeta, int Pmax,
-                   const int *d_hboff, size_t half, void *d_Hb, void *stream);
+/* split-BF16 copies of the standard rotation blocks H_n of beta b (from
+ * the FP64 blocks): at hboff[n] (rows padded to 16, K to 32, row-major), hi
+ * part at b * 2 half, lo part half values further */
+int fmbh_split_rot(const double *d_H64, int nbeta, int Pmax, const int *d_hboff, size_t half,
+                   void *d_Hb, void *stream);
 int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pin,
                   int pout, const int *d_coff, size_t pstride, void *d_Cp,
                   void *stream);

@@ -18,10 +18,18 @@
 // (pair x right-hand side) is one block-diagonal GEMM per stage:
 //   * X[f][col], f = n^2+n+m, col = 2*vector + {re, im} (16*NT real columns,
 //     rows padded by 2 for the LDS banks);
-//   * rotation block n: X_n <- H_n X_n, a real (s x s)(s x 16NT) product on
-//     v_mfma_{f32,f64}_16x16x4 (A = H from global memory, B = X rows);
+//   * rotation of degree n: H_n^{m'm} = H_n^{-m',-m}, so in the even/odd
+//     coordinates e_m = (z_m + z_{-m}) / 2 (e_0 = z_0) and o_m = (z_m -
+//     z_{-m}) / 2 it splits into an even block E_n ((n+1) x (n+1)) and an
+//     odd block O_n (n x n), about half the products of H_n: real (s x s)(s
+//     x 16NT) products on v_mfma_{f32,f64}_16x16x4 (A = E_n, O_n from global
+//     memory, B = X rows).  The first rotation forms the coordinates from
+//     the rows +-m as it reads them and leaves e_m in row n^2+n+m, o_m in
+//     row n^2+n-m; the back rotation recombines z_{+-m} = e_m +- o_m in its
+//     accumulators;
 //   * coaxial block m: X_m <- C_m X_m, complex, as Cr*B + Ci*B' with
-//     B' = (-im, re) formed from the same rows.
+//     B' = (-im, re) formed from the same rows.  C^{|m|} acts alike on the
+//     rows +m and -m, so it maps even/odd coordinates to even/odd ones.
 // The operator blocks are stored zero-padded to 16-row / 4-column multiples
 // (column-major, leading dimension = padded rows), so operand loads need no
 // guards: rows of X beyond a block are read at a clamped index and meet
@@ -120,77 +128,138 @@ struct RcrArgs {
   void *gws;           // GWS mode: per-workgroup 2 x Pm^2 x RS reals
 };
 
-// Row tiles [t0, t0 + NTL) of rotation block n (size s, first row base):
-// Xo[rows] = Hn Xi[rows].  Ab: the block (padded, leading dimension pad16(s)).
-template <typename T, int NT, int NTL>
-__device__ __forceinline__ void rot_unit(const T *Xi, T *Xo,
-                                         const T *__restrict__ Ab, int s,
-                                         int base, int t0) {
+// values of a padded even/odd rotation block of s rows (16-row / 4-column
+// multiples)
+__device__ __forceinline__ int eo_block(int s) { return ((s + 15) & ~15) * ((s + 3) & ~3); }
+
+// Row tiles [t0, t0 + NTL) of degree n's rotation in even/odd coordinates.
+// Index i of the even block (n + 1 rows and columns) stands for m_i = i + 1
+// (i < n) and m_n = 0; index i of the odd block (n x n) for |m| = i + 1, so
+// that even and odd row i of a tile sit in the same lane and element.
+//   DIR 0 (first rotation): the batch rows hold standard coordinates z_m;
+//     the operands are e = z_m + z_{-m} and o = z_m - z_{-m} (so e_0 = 2 z_0),
+//     and the outputs go to the rows of e_m (n^2 + n + m) and o_m
+//     (n^2 + n - m): even/odd coordinates for the coaxial stage.
+//   DIR 1 (back rotation): the rows hold even/odd coordinates; the outputs
+//     are recombined to y_{+-m} = e_m +- o_m, y_0 = e_0, in standard rows.
+// Both blocks run in one K loop (their operand streams in flight together).
+// Ab: the even block (padded, leading dimension pad16(n + 1)); the odd block
+// follows it.  The odd block has NTL or NTL - 1 of the tiles (none for n = 0)
+// and no more K-steps than the even one; its missing tiles and K-steps are
+// skipped (wave-uniform) and its operands read at clamped, in-bounds
+// positions.
+template <typename T, int NT, int NTL, int DIR>
+__device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict__ Ab,
+                                         int n, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
   constexpr int RS = Row<NT>::stride, KU = Ahead<T, NTL>::ku;
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  const int ld = pad16(s), nks = (s + 3) >> 2, ng = (nks + KU - 1) / KU;
-  const T *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  const int se = n + 1, so = n, c0 = n * n + n;  // c0: the row of m = 0
+  const int lde = pad16(se), ldo = pad16(so > 0 ? so : 1);
+  const int nke = (se + 3) >> 2, nko = (so + 3) >> 2, ng = (nke + KU - 1) / KU;
+  const int nto = min(NTL, ((so + 15) >> 4) - t0);  // odd tiles of this unit
+  const T *ape = Ab + (unsigned)(lk * lde + 16 * t0 + li);
+  // (a unit past the odd block's tiles reads the even block's operands)
+  const T *apo = nto > 0 ? Ab + (unsigned)(eo_block(se) + lk * ldo + 16 * t0 + li) : ape;
   // set by the first K-step (every block has one), which multiplies into a
   // zero accumulator operand instead of cleared registers
-  acc_t acc[NTL][NT];
-  T a0[KU][NTL], b0[KU][NT], a1[KU][NTL], b1[KU][NT];
-  auto fetch = [&](int g, T(&fa)[KU][NTL], T(&fb)[KU][NT]) {
+  acc_t acce[NTL][NT], acco[NTL][NT];
+  T ae0[KU][NTL], ao0[KU][NTL], be0[KU][NT], bo0[KU][NT];
+  T ae1[KU][NTL], ao1[KU][NTL], be1[KU][NT], bo1[KU][NT];
+  auto fetch = [&](int g, T(&fae)[KU][NTL], T(&fao)[KU][NTL], T(&fbe)[KU][NT],
+                   T(&fbo)[KU][NT]) {
 #pragma unroll
     for (int u = 0; u < KU; u++) {
-      const int ks = min(g * KU + u, nks - 1);
-      const int k = min(4 * ks + lk, s - 1);
-      const T *xr = Xi + (base + k) * RS + li;
+      const int ks = min(g * KU + u, nke - 1);
+      const int k = min(4 * ks + lk, n), m = k < n ? k + 1 : 0;
+      const T *xp = Xi + (c0 + m) * RS + li, *xm = Xi + (c0 - m) * RS + li;
 #pragma unroll
-      for (int t = 0; t < NT; t++) fb[u][t] = xr[16 * t];
-      const T *ak = ap + (unsigned)(4 * ks * ld);
+      for (int t = 0; t < NT; t++) {
+        const T p = xp[16 * t], q = xm[16 * t];
+        if constexpr (DIR == 0) {
+          fbe[u][t] = p + q;
+          fbo[u][t] = p - q;
+        } else {
+          fbe[u][t] = p;
+          fbo[u][t] = q;
+        }
+      }
+      const T *ak = ape + (unsigned)(4 * ks * lde);
 #pragma unroll
-      for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
+      for (int mt = 0; mt < NTL; mt++) fae[u][mt] = ak[16 * mt];
+      const int kso = min(ks, max(nko - 1, 0));
+      const T *ao = apo + (unsigned)(4 * kso * ldo);
+#pragma unroll
+      for (int mt = 0; mt < NTL; mt++) fao[u][mt] = ao[16 * (mt < nto ? mt : 0)];
     }
   };
-  auto mma = [&](int g, const T(&fa)[KU][NTL], const T(&fb)[KU][NT]) {
+  auto mma = [&](int g, const T(&fae)[KU][NTL], const T(&fao)[KU][NTL],
+                 const T(&fbe)[KU][NT], const T(&fbo)[KU][NT]) {
 #pragma unroll
     for (int u = 0; u < KU; u++) {
-      if (u == 0 && g == 0) {
+      const int ks = g * KU + u;
+      if (ks == 0) {
 #pragma unroll
         for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
           for (int t = 0; t < NT; t++)
-            acc[mt][t] = Mfma<T>::mma(fa[u][mt], fb[u][t], acc_t{0, 0, 0, 0});
-      } else if (g * KU + u < nks) {
+            acce[mt][t] = Mfma<T>::mma(fae[u][mt], fbe[u][t], acc_t{0, 0, 0, 0});
+      } else if (ks < nke) {
 #pragma unroll
         for (int mt = 0; mt < NTL; mt++)
 #pragma unroll
-          for (int t = 0; t < NT; t++) acc[mt][t] = Mfma<T>::mma(fa[u][mt], fb[u][t], acc[mt][t]);
+          for (int t = 0; t < NT; t++) acce[mt][t] = Mfma<T>::mma(fae[u][mt], fbe[u][t], acce[mt][t]);
       }
+#pragma unroll
+      for (int mt = 0; mt < NTL; mt++)
+        if (mt < nto) {
+          if (ks == 0) {
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+              acco[mt][t] = Mfma<T>::mma(fao[u][mt], fbo[u][t], acc_t{0, 0, 0, 0});
+          } else if (ks < nko) {
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+              acco[mt][t] = Mfma<T>::mma(fao[u][mt], fbo[u][t], acco[mt][t]);
+          }
+        }
     }
   };
-  fetch(0, a0, b0);
+  fetch(0, ae0, ao0, be0, bo0);
   for (int g = 0; g < ng; g += 2) {
-    if (g + 1 < ng) fetch(g + 1, a1, b1);
-    mma(g, a0, b0);
+    if (g + 1 < ng) fetch(g + 1, ae1, ao1, be1, bo1);
+    mma(g, ae0, ao0, be0, bo0);
     if (g + 1 < ng) {
-      if (g + 2 < ng) fetch(g + 2, a0, b0);
-      mma(g + 1, a1, b1);
+      if (g + 2 < ng) fetch(g + 2, ae0, ao0, be0, bo0);
+      mma(g + 1, ae1, ao1, be1, bo1);
     }
   }
-  // rows of a full tile are stored unguarded (a wave-uniform test)
+  // tile row i of both blocks: |m| = i + 1 (i < n, where the odd block has
+  // the tile too) or m = 0 (i = n, even block only); padding rows are skipped
 #pragma unroll
   for (int mt = 0; mt < NTL; mt++) {
-    const int r0 = 16 * (t0 + mt);
-    T *xo = Xo + (base + r0) * RS + li;
-    if (r0 + 16 <= s) {
+    const bool odd = mt < nto;
 #pragma unroll
-      for (int e = 0; e < 4; e++)
+    for (int e = 0; e < 4; e++) {
+      const int i = 16 * (t0 + mt) + Mfma<T>::row(lk, e);
+      if (i > n) continue;
+      const int m = i < n ? i + 1 : 0;
+      T *yp = Xo + (c0 + m) * RS + li, *ym = Xo + (c0 - m) * RS + li;
+      if (i == n || !odd) {
 #pragma unroll
-        for (int t = 0; t < NT; t++) xo[Mfma<T>::row(lk, e) * RS + 16 * t] = acc[mt][t][e];
-    } else {
+        for (int t = 0; t < NT; t++) yp[16 * t] = acce[mt][t][e];
+      } else if constexpr (DIR == 0) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int i = Mfma<T>::row(lk, e);
-        if (r0 + i < s) {
+        for (int t = 0; t < NT; t++) {
+          yp[16 * t] = acce[mt][t][e];
+          ym[16 * t] = acco[mt][t][e];
+        }
+      } else {
 #pragma unroll
-          for (int t = 0; t < NT; t++) xo[i * RS + 16 * t] = acc[mt][t][e];
+        for (int t = 0; t < NT; t++) {
+          const T ev = acce[mt][t][e], ov = acco[mt][t][e];
+          yp[16 * t] = ev + ov;
+          ym[16 * t] = ev - ov;
         }
       }
     }
@@ -297,16 +366,16 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
 }
 
 // runs unit<NTL = ntl> for a wave-uniform ntl in [1, MT]
-template <typename T, int NT, int MT>
-__device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo,
-                                        const T *Ab, int s, int base, int t0) {
+template <typename T, int NT, int MT, int DIR>
+__device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo, const T *Ab, int n,
+                                        int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      rot_any<T, NT, MT - 1>(ntl, Xi, Xo, Ab, s, base, t0);
+      rot_any<T, NT, MT - 1, DIR>(ntl, Xi, Xo, Ab, n, t0);
       return;
     }
   }
-  rot_unit<T, NT, MT>(Xi, Xo, Ab, s, base, t0);
+  rot_unit<T, NT, MT, DIR>(Xi, Xo, Ab, n, t0);
 }
 
 template <typename T, int NT, int MT, int SG>
@@ -323,16 +392,15 @@ __device__ __forceinline__ void coax_any(int ntl, const T *Xi, T *Xo,
 }
 
 // Rotation stage over this wave's units {n*16 + g, operator offset}: row
-// tiles [g*MT, g*MT + MT) of block n.
-template <typename T, int NT, int MT>
+// tiles [g*MT, g*MT + MT) of degree n's even and odd blocks.
+template <typename T, int NT, int MT, int DIR>
 __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
                                           const int *units, int count) {
   for (int j = 0; j < count; j++) {
     const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
     const int n = code >> 4, t0 = (code & 15) * MT;
-    const int s = 2 * n + 1;
-    const int ntl = min(MT, ((s + 15) >> 4) - t0);
-    rot_any<T, NT, MT>(ntl, Xi, Xo, H + aoff, s, n * n, t0);
+    const int ntl = min(MT, ((n + 1 + 15) >> 4) - t0);
+    rot_any<T, NT, MT, DIR>(ntl, Xi, Xo, H + aoff, n, t0);
   }
 }
 
@@ -343,9 +411,11 @@ __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
 // three terms hi.hi + hi.lo + lo.hi (relative error about 2^-16).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// Row tiles [t0, t0 + NTL) of rotation block n (size s, first row base) in
-// split-BF16: Ab is the block's hi part (rows padded to 16, K to 32,
-// row-major); its lo part lies `half` values further on.
+// Row tiles [t0, t0 + NTL) of the standard rotation block n (size s = 2n+1,
+// first row base = n^2) in split-BF16, on the same batch rows as the
+// even/odd units (standard coordinates in, standard out): Ab is the block's
+// hi part (rows padded to 16, K to 32, row-major); its lo part lies `half`
+// values further on.
 template <int NT, int NTL>
 __device__ __forceinline__ void rot_unit_bf(const float *Xi, float *Xo,
                                             const __bf16 *__restrict__ Ab, size_t half,
@@ -468,15 +538,26 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
   }
 }
 
-// One batch: vectors u0 .. u0+V-1 of chunk ch.
+// Value x of lane l (wave-uniform) as a scalar.
+__device__ __forceinline__ long long lane64(long long x, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), l);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// One batch: vectors u0 .. u0+V-1 of chunk ch (u0 below the chunk's count).
+// pre() runs just before the first barrier (the LDS-mode kernel stores its
+// schedule there, so that the schedule's loads and the source gather are in
+// flight together).
 // OPL: the bucket's operators are Hs / Cs (LDS copies) instead of global
-template <typename T, int NT, int MT, int W, bool GWS, bool BF = false, bool OPL = false>
+template <typename T, int NT, int MT, int W, bool GWS, bool BF = false, bool OPL = false,
+          typename Pre>
 __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
-                                          T *X0, T *X1, cplx<T> *ph,
-                                          const int *sched, const T *Hs = nullptr,
+                                          T *X0, T *X1, const int *sched, Pre pre,
+                                          const T *Hs = nullptr,
                                           const cplx<T> *Cs = nullptr) {
-  constexpr int V = 8 * NT, RS = Row<NT>::stride;
-  const int tid = threadIdx.x, wave = uni(tid >> 6);
+  constexpr int V = 8 * NT, RS = Row<NT>::stride, NTH = kThreads<W>;
+  const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
   const int pin = a.pin, pout = a.pout;
   const int Pm = pin > pout ? pin : pout;
   const int Pi2 = pin * pin, Po2 = pout * pout, Pm2 = Pm * Pm;
@@ -490,75 +571,97 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
     H = (const T *)a.H + a.hstride * oi.beta;
     C = (const cplx<T> *)a.C + a.cstride * oi.dist;
   }
-  const int U = (ch.z - ch.y) * a.nrhs;
-  __syncthreads();
+  // e^{i m phi}, m < Pm, read through the caches where needed
   const cplx<T> *phb = (const cplx<T> *)a.phase + (size_t)a.pstride * ch.x;
-  for (int m = tid; m < Pm; m += kThreads<W>) ph[m] = phb[m];
-  // vector v of the batch: (pair, right-hand side) = divmod(u0 + v, nrhs);
-  // its source and target base pointers are wave-uniform
-  const cplx<T> *vin[V];
-  cplx<T> *vout[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    const int u = u0 + v;
-    vin[v] = nullptr;
-    vout[v] = nullptr;
-    if (u < U) {
-      const int q = ch.y + u / a.nrhs, r = u % a.nrhs;
-      vin[v] = (const cplx<T> *)a.Min + ((size_t)a.psrc[q] * a.nrhs + r) * Pi2;
-      vout[v] = (cplx<T> *)a.Lout + ((size_t)a.ptgt[q] * a.nrhs + r) * Po2;
-    }
+  // vector v of the batch: (pair, right-hand side) = divmod(u0 + v, nrhs),
+  // present for v < nv.  Lane v of every wave computes vector v's source and
+  // target offsets (complex values) with one load of each index array; the
+  // loops read them back as wave-uniform values.  Lanes past nv repeat
+  // vector 0, so every source address below is in bounds; the columns of
+  // absent vectors are computed but never added to a target (a product's
+  // column depends only on the same column of its input).
+  const int nv = min(V, (ch.z - ch.y) * a.nrhs - u0);
+  long long soff, toff;
+  {
+    const int u = u0 + (lane < nv ? lane : 0);
+    const int qq = u / a.nrhs, r = u - qq * a.nrhs, q = ch.y + qq;
+    soff = ((long long)a.psrc[q] * a.nrhs + r) * Pi2;
+    toff = ((long long)a.ptgt[q] * a.nrhs + r) * Po2;
   }
-  __syncthreads();
-  // source coefficients times e^{i m phi}; rows f >= pin^2 and absent
-  // vectors are zero.  One thread per row f: its V loads are in flight
-  // together, the degree and phase of f are computed once.
-  for (int f = tid; f < Pm2; f += kThreads<W>) {
-    cplx<T> z[V];
+  const cplx<T> *vin[V];
 #pragma unroll
-    for (int v = 0; v < V; v++)
-      z[v] = (vin[v] && f < Pi2) ? vin[v][(unsigned)f] : cmk<T>(0, 0);
-    const int n = isqrt_fix(f), m = f - n * n - n;
-    cplx<T> w = ph[m < 0 ? -m : m];
-    if (m < 0) w.im = -w.im;
+  for (int v = 0; v < V; v++) vin[v] = (const cplx<T> *)a.Min + lane64(soff, v);
+  // source coefficients times e^{i m phi} (times 1/2 for the even/odd
+  // rotations, whose first stage forms z_m + z_{-m} and z_m - z_{-m}); rows
+  // f >= pin^2 are zero.  Rows f = tid and f = tid + NTH are loaded (at
+  // clamped, in-bounds rows) before the barrier that frees X0 from the
+  // previous batch; larger orders loop.
+  constexpr T kScale = BF ? T(1) : T(0.5);
+  auto row = [&](int f, cplx<T> (&z)[V], cplx<T> &w, int &m) {
+    const int g = min(f, Pi2 - 1), h = min(f, Pm2 - 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) z[v] = vin[v][(unsigned)g];
+    const int n = isqrt_fix(h);
+    m = h - n * n - n;
+    w = phb[m < 0 ? -m : m];
+  };
+  auto put = [&](int f, const cplx<T> (&z)[V], cplx<T> w, int m) {
+    w = cmk<T>(kScale * w.re, m < 0 ? -kScale * w.im : kScale * w.im);
+    if (f >= Pi2) w = cmk<T>(0, 0);
 #pragma unroll
     for (int v = 0; v < V; v++) *(cplx<T> *)(X0 + f * RS + 2 * v) = cmul(z[v], w);
+  };
+  const bool two = Pm2 > NTH;  // wave-uniform
+  cplx<T> z0[V], z1[V], w0, w1;
+  int m0, m1;
+  row(tid, z0, w0, m0);
+  if (two) row(tid + NTH, z1, w1, m1);
+  pre();
+  __syncthreads();
+  if (tid < Pm2) put(tid, z0, w0, m0);
+  if (two && tid + NTH < Pm2) put(tid + NTH, z1, w1, m1);
+  for (int f = tid + 2 * NTH; f < Pm2; f += NTH) {
+    row(f, z0, w0, m0);
+    put(f, z0, w0, m0);
   }
   __syncthreads();
   const int *o1 = sched, *oc = sched + W + 1, *o3 = sched + 2 * (W + 1);
-  const int *u1 = sched + 2 * uni(o1[wave]), w1 = uni(o1[wave + 1] - o1[wave]);
-  const int *uc = sched + 2 * uni(oc[wave]), wc = uni(oc[wave + 1] - oc[wave]);
-  const int *u3 = sched + 2 * uni(o3[wave]), w3 = uni(o3[wave + 1] - o3[wave]);
+  const int *u1 = sched + 2 * uni(o1[wave]), w1c = uni(o1[wave + 1] - o1[wave]);
+  const int *uc = sched + 2 * uni(oc[wave]), wcc = uni(oc[wave + 1] - oc[wave]);
+  const int *u3 = sched + 2 * uni(o3[wave]), w3c = uni(o3[wave + 1] - o3[wave]);
   if constexpr (BF) {
     const __bf16 *Hb = (const __bf16 *)a.Hb + a.hbstride * oi.beta;
-    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u1, w1);
+    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
+    coax_stage<T, NT, MT>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
-    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u3, w3);
+    rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u3, w3c);
   } else {
-    rot_stage<T, NT, MT>(X0, X1, H, u1, w1);
+    rot_stage<T, NT, MT, 0>(X0, X1, H, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT>(X1, X0, C, uc, wc, pin, pout);
+    coax_stage<T, NT, MT>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
-    rot_stage<T, NT, MT>(X0, X1, H, u3, w3);
+    rot_stage<T, NT, MT, 1>(X0, X1, H, u3, w3c);
   }
   __syncthreads();
   // add e^{-i m' phi} y into the targets.  Thread e adds real e of each
   // target vector (part e & 1 of coefficient e >> 1), so that one wave's
   // atomic adds fill whole cache lines: the adds are performed beyond the
   // L2, and half-filled lines cost nearly as much as full ones.
-  for (int e = tid; e < 2 * Po2; e += kThreads<W>) {
+  T *vout[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) vout[v] = (T *)((cplx<T> *)a.Lout + lane64(toff, v));
+  for (int e = tid; e < 2 * Po2; e += NTH) {
     const int f = e >> 1, n = isqrt_fix(f), m = f - n * n - n;
     const bool im = e & 1;
-    cplx<T> w = ph[m < 0 ? -m : m];
+    cplx<T> w = phb[m < 0 ? -m : m];
     if (m > 0) w.im = -w.im;
 #pragma unroll
     for (int v = 0; v < V; v++)
-      if (vout[v]) {
+      if (v < nv) {
         const cplx<T> x = *(const cplx<T> *)(X1 + f * RS + 2 * v);
         const T y = im ? x.re * w.im + x.im * w.re : x.re * w.re - x.im * w.im;
-        unsafeAtomicAdd((T *)vout[v] + (unsigned)e, y);
+        unsafeAtomicAdd(vout[v] + (unsigned)e, y);
       }
   }
 }
@@ -580,21 +683,29 @@ __global__ void __launch_bounds__(kThreads<W>)
     // workgroups are dealt to the 8 XCDs round-robin: workgroup g runs on
     // XCD g % 8, which takes the contiguous item range x * per .. x * per +
     // per - 1, so that neighbouring chunks share the XCD's L2
-    const long long items = (long long)a.nchunks * a.nb;
-    const long long per = (items + 7) / 8;
-    const long long it = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    // (the launch keeps the grid, so the item count, below 2^31)
+    const unsigned items = (unsigned)a.nchunks * (unsigned)a.nb;
+    const unsigned per = (items + 7) / 8;
+    const unsigned it = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (it >= items || (blockIdx.x >> 3) >= per) return;
-    cx = (int)(it / a.nb);
-    by = (int)(it % a.nb);
+    cx = (int)(it / (unsigned)a.nb);
+    by = (int)(it % (unsigned)a.nb);
   }
   const int4 ch = a.chunks[cx];
   const int u0 = by * V;
   if (u0 >= (ch.z - ch.y) * a.nrhs) return;
   T *X = (T *)smem;
-  cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * RS);
-  int *sched = (int *)(ph + Pm + (Pm & 1));
-  for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
-  rcr_batch<T, NT, MT, W, false, BF>(a, ch, u0, X, X, ph, sched);
+  int *sched = (int *)(X + (size_t)Pm * Pm * RS);
+  // the schedule's first 2 NTH entries are loaded here and stored by
+  // rcr_batch just before its first barrier, with the gather in flight
+  constexpr int NTH = kThreads<W>;
+  const int t = threadIdx.x, n = a.sched_len;
+  const int s0 = t < n ? a.sched[t] : 0, s1 = t + NTH < n ? a.sched[t + NTH] : 0;
+  rcr_batch<T, NT, MT, W, false, BF>(a, ch, u0, X, X, sched, [&] {
+    if (t < n) sched[t] = s0;
+    if (t + NTH < n) sched[t + NTH] = s1;
+    for (int i = t + 2 * NTH; i < n; i += NTH) sched[i] = a.sched[i];
+  });
 }
 
 // LDS mode on a persistent grid (one workgroup per free slot of the GPU):
@@ -607,15 +718,14 @@ __global__ void __launch_bounds__(kThreads<W>)
   extern __shared__ __align__(16) unsigned char smem[];
   const int Pm = a.pin > a.pout ? a.pin : a.pout;
   T *X = (T *)smem;
-  cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * Row<NT>::stride);
-  int *sched = (int *)(ph + Pm + (Pm & 1));
+  int *sched = (int *)(X + (size_t)Pm * Pm * Row<NT>::stride);
   for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
   const long long items = (long long)a.nchunks * a.nb;
   for (long long it = blockIdx.x; it < items; it += gridDim.x) {
     const int c = (int)(it / a.nb), b = (int)(it % a.nb);
     const int4 ch = a.chunks[c];
     if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
-    rcr_batch<T, NT, MT, W, false, BF>(a, ch, b * V, X, X, ph, sched);
+    rcr_batch<T, NT, MT, W, false, BF>(a, ch, b * V, X, X, sched, [] {});
   }
 }
 
@@ -635,8 +745,7 @@ __global__ void __launch_bounds__(kThreads<W>)
   extern __shared__ __align__(16) unsigned char smem[];
   const int Pm = a.pin > a.pout ? a.pin : a.pout;
   T *X = (T *)smem;
-  cplx<T> *ph = (cplx<T> *)(X + (size_t)Pm * Pm * Row<NT>::stride);
-  int *sched = (int *)(ph + Pm + (Pm & 1));
+  int *sched = (int *)(X + (size_t)Pm * Pm * Row<NT>::stride);
   T *Hs = (T *)(smem + align16((size_t)((unsigned char *)(sched + a.sched_len) - smem)));
   cplx<T> *Cs = (cplx<T> *)(Hs + a.hlen + (a.hlen & 1));
   for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
@@ -665,7 +774,7 @@ __global__ void __launch_bounds__(kThreads<W>)
       dist = oi.dist;
       // rcr_batch opens with a barrier, which publishes the copies
     }
-    rcr_batch<T, NT, MT, W, false, false, true>(a, ch, b * V, X, X, ph, sched, Hs, Cs);
+    rcr_batch<T, NT, MT, W, false, false, true>(a, ch, b * V, X, X, sched, [] {}, Hs, Cs);
   }
 }
 
@@ -678,20 +787,22 @@ __global__ void __launch_bounds__(kThreads<W>) k_rcr_gws(RcrArgs a) {
   const size_t slice = (size_t)Pm * Pm * RS;
   T *X0 = (T *)a.gws + 2 * slice * blockIdx.x;
   T *X1 = X0 + slice;
-  cplx<T> *ph = (cplx<T> *)smem;
-  int *sched = (int *)(ph + Pm + (Pm & 1));
+  int *sched = (int *)smem;
   for (int i = threadIdx.x; i < a.sched_len; i += kThreads<W>) sched[i] = a.sched[i];
   const long long items = (long long)a.nchunks * a.nb;
   for (long long it = blockIdx.x; it < items; it += gridDim.x) {
     const int c = (int)(it / a.nb), b = (int)(it % a.nb);
     const int4 ch = a.chunks[c];
     if (b * V >= (ch.z - ch.y) * a.nrhs) continue;
-    rcr_batch<T, NT, MT, W, true>(a, ch, b * V, X0, X1, ph, sched);
+    rcr_batch<T, NT, MT, W, true>(a, ch, b * V, X0, X1, sched, [] {});
   }
 }
 
-inline int mt_lds(int pm) {
-  int tiles = (2 * pm - 1 + 15) / 16;
+// row tiles of the largest whole block (an even rotation block or a coaxial
+// block: at most pm rows; a split-BF16 standard rotation block: 2 pm - 1),
+// or 0 beyond the LDS-mode units
+inline int mt_lds(int pm, int bf = 0) {
+  int tiles = ((bf ? 2 * pm - 1 : pm) + 15) / 16;
   if (tiles <= 2) return 2;
   if (tiles <= 3) return 3;
   if (tiles <= 4) return 4;
@@ -699,9 +810,11 @@ inline int mt_lds(int pm) {
   return 0;
 }
 
-// phase table and a schedule of sched_len ints
+// the schedule: sched_len ints
 inline size_t small_lds(int pm, size_t tsz, int sched_len) {
-  return (size_t)(pm + (pm & 1)) * 2 * tsz + sizeof(int) * (size_t)sched_len;
+  (void)pm;
+  (void)tsz;
+  return sizeof(int) * (size_t)sched_len;
 }
 
 
@@ -709,7 +822,7 @@ inline size_t x_bytes(int pm, int nt, size_t tsz) {
   return (size_t)pm * pm * (16 * nt + 2) * tsz;
 }
 
-// LDSO mode: batch, phases, schedule, then (16-byte aligned) hlen rotation
+// LDSO mode: batch, schedule, then (16-byte aligned) hlen rotation
 // values and clen complex coaxial values
 inline size_t ldso_bytes(int pm, int nt, size_t tsz, int sched_len, int hlen, size_t clen) {
   size_t b = (x_bytes(pm, nt, tsz) + small_lds(pm, tsz, sched_len) + 15) & ~(size_t)15;

@@ -5,32 +5,61 @@
 
 namespace {
 
-// zero-padded copies of the operator blocks (see the file comment)
+// Even/odd rotation blocks of degree n from the FP64 blocks H_n (column-
+// major s x s, s = 2n+1, H_n^{m'm} at [(m+n) s + m'+n]), rounded once to T.
+// With H_n^{m'm} = H_n^{-m',-m} and the coordinates e_m = (z_m + z_{-m})/2
+// (m > 0), e_0 = z_0, o_m = (z_m - z_{-m})/2 (m > 0), y = H_n z becomes
+//   e(y)_{m'} = sum_m E_{m'm} e(z)_m,  E_{m'0} = H^{m'0},
+//                                     E_{m'm} = H^{m'm} + H^{m',-m} (m > 0);
+//   o(y)_{m'} = sum_m O_{m'm} o(z)_m,  O_{m'm} = H^{m'm} - H^{m',-m}.
+// E's index i stands for m = i + 1 (i < n) and m = 0 (i = n), O's for
+// |m| = i + 1, so that E and O row i of a tile meet in the same lane.  Each
+// block zero-padded to 16 rows / 4 columns, column-major with leading
+// dimension the padded rows; E then O.
 template <typename T>
-__global__ void k_pad_rot(const T *__restrict__ H, size_t hs, const int *__restrict__ hoff,
+__global__ void k_pad_rot(const double *__restrict__ H, size_t hs, const int *__restrict__ hoff,
                           size_t ps, T *__restrict__ Hp) {
   const int n = blockIdx.x, b = blockIdx.y;
-  const int s = 2 * n + 1, ld = (s + 15) & ~15, nc = (s + 3) & ~3;
-  const T *src = H + hs * b + rot_off(n);
+  const int s = 2 * n + 1;
+  const double *src = H + hs * b + rot_off(n);
+  auto h = [&](int mp, int m) { return src[(size_t)(m + n) * s + (mp + n)]; };
   T *dst = Hp + ps * b + hoff[n];
-  for (int e = threadIdx.x; e < ld * nc; e += blockDim.x) {
-    const int i = e % ld, k = e / ld;
-    dst[e] = (i < s && k < s) ? src[k * s + i] : T(0);
+  {
+    const int se = n + 1, ld = (se + 15) & ~15, nc = (se + 3) & ~3;
+    for (int e = threadIdx.x; e < ld * nc; e += blockDim.x) {
+      const int i = e % ld, k = e / ld;
+      double v = 0.0;
+      if (i < se && k < se) {
+        const int mp = i < n ? i + 1 : 0, m = k < n ? k + 1 : 0;
+        v = m ? h(mp, m) + h(mp, -m) : h(mp, 0);
+      }
+      dst[e] = (T)v;
+    }
+  }
+  if (n > 0) {
+    const int so = n, ld = (so + 15) & ~15, nc = (so + 3) & ~3;
+    T *od = dst + eo_block(n + 1);
+    for (int e = threadIdx.x; e < ld * nc; e += blockDim.x) {
+      const int i = e % ld, k = e / ld;
+      double v = 0.0;
+      if (i < so && k < so) v = h(i + 1, k + 1) - h(i + 1, -(k + 1));
+      od[e] = (T)v;
+    }
   }
 }
 
-// split-BF16 rotation blocks from the FP32 padded ones (column-major,
-// leading dimension pad16(s)): row-major, K padded to 32, hi = bf16(h),
+// split-BF16 copies of the standard rotation blocks H_n (from FP64, via the
+// FP32 value h): row-major, rows padded to 16, K to 32, hi = bf16(h),
 // lo = bf16(h - hi)
-__global__ void k_split_rot(const float *__restrict__ Hp, size_t ps, const int *__restrict__ hoff,
+__global__ void k_split_rot(const double *__restrict__ H, size_t hs,
                             const int *__restrict__ hboff, size_t half, __bf16 *__restrict__ Hb) {
   const int n = blockIdx.x, b = blockIdx.y;
   const int s = 2 * n + 1, ld = (s + 15) & ~15, kp = (s + 31) & ~31;
-  const float *src = Hp + ps * b + hoff[n];
+  const double *src = H + hs * b + rot_off(n);
   __bf16 *hi = Hb + 2 * half * b + hboff[n];
   for (int e = threadIdx.x; e < ld * kp; e += blockDim.x) {
     const int r = e / kp, k = e % kp;
-    const float v = (r < s && k < s) ? src[k * ld + r] : 0.0f;
+    const float v = (r < s && k < s) ? (float)src[(size_t)k * s + r] : 0.0f;
     const __bf16 h = (__bf16)v;
     hi[e] = h;
     hi[e + half] = (__bf16)(v - (float)h);
@@ -80,7 +109,7 @@ extern "C" size_t fmbh_rcr_lds_bytes(int pm, int prec, int nt, int sched_len) {
   return x_bytes(pm, nt, tsz) + small_lds(pm, tsz, sched_len);
 }
 
-extern "C" int fmbh_rcr_mt_lds(int pm) { return mt_lds(pm); }
+extern "C" int fmbh_rcr_mt_lds(int pm, int bf) { return mt_lds(pm, bf); }
 
 extern "C" size_t fmbh_rcr_ldso_bytes(int pm, int prec, int nt, int sched_len, int hlen,
                                       size_t clen) {
@@ -109,17 +138,17 @@ extern "C" int fmbh_rcr(const fmbh_rcr_job *j, int prec, int nrhs, void *stream)
   return j->nw == 8 ? run_f32_w8(j, nrhs, st) : run_f32_w4(j, nrhs, st);
 }
 
-extern "C" int fmbh_pad_rot(int prec, const void *d_H, int nbeta, int Pmax,
+extern "C" int fmbh_pad_rot(int prec, const double *d_H64, int nbeta, int Pmax,
                             const int *d_hoff, size_t pstride, void *d_Hp,
                             void *stream) {
   hipStream_t st = (hipStream_t)stream;
   size_t hs = (size_t)rot_off(Pmax);
   if (prec == FMB_FP64)
     hipLaunchKernelGGL(k_pad_rot<double>, dim3(Pmax, nbeta), dim3(256), 0, st,
-                       (const double *)d_H, hs, d_hoff, pstride, (double *)d_Hp);
+                       d_H64, hs, d_hoff, pstride, (double *)d_Hp);
   else
     hipLaunchKernelGGL(k_pad_rot<float>, dim3(Pmax, nbeta), dim3(256), 0, st,
-                       (const float *)d_H, hs, d_hoff, pstride, (float *)d_Hp);
+                       d_H64, hs, d_hoff, pstride, (float *)d_Hp);
   FMB_LAUNCH_CHECK();
   return FMB_OK;
 }
@@ -142,11 +171,10 @@ extern "C" int fmbh_pad_coax(int prec, const void *d_C, int ncoax, int P, int pi
   return FMB_OK;
 }
 
-extern "C" int fmbh_split_rot(const void *d_Hp, size_t pstride, const int *d_hoff, int nbeta,
-                              int Pmax, const int *d_hboff, size_t half, void *d_Hb,
-                              void *stream) {
-  hipLaunchKernelGGL(k_split_rot, dim3(Pmax, nbeta), dim3(256), 0, (hipStream_t)stream,
-                     (const float *)d_Hp, pstride, d_hoff, d_hboff, half, (__bf16 *)d_Hb);
+extern "C" int fmbh_split_rot(const double *d_H64, int nbeta, int Pmax, const int *d_hboff,
+                              size_t half, void *d_Hb, void *stream) {
+  hipLaunchKernelGGL(k_split_rot, dim3(Pmax, nbeta), dim3(256), 0, (hipStream_t)stream, d_H64,
+                     (size_t)rot_off(Pmax), d_hboff, half, (__bf16 *)d_Hb);
   FMB_LAUNCH_CHECK();
   return FMB_OK;
 }

@@ -39,6 +39,7 @@ void fmb_plan_params_default(fmb_plan_params *p) {
   p->nedge = 24;
   p->nrhs_max = 1;
   p->m2l_tile = 256; /* measured: level-7 M2L 70.0 -> 66.5 ms (FP32), 128 -> 116 ms (FP64) */
+  p->m2l_tile_min_mb = 256;
 }
 
 static const int k_dist2[FMB_NDIST] = {4, 5, 6, 8, 9, 10, 11, 12,
@@ -502,6 +503,9 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
   if (prm->waves != 0 && prm->waves != 4 && prm->waves != 8)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: waves %d (0, 4 or 8)", prm->waves);
+  if (prm->m2l_tile < 0 || prm->m2l_tile_min_mb < 0)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: m2l_tile %d, m2l_tile_min_mb %d (>= 0)",
+                         prm->m2l_tile, prm->m2l_tile_min_mb);
   if (prm->split_bf16 && (prm->precision != FMB_MIXED || prm->rcr_path != FMB_RCR_AUTO))
     return fmb_set_error(FMB_ERR_ARG,
                          "fmb_plan_create: split_bf16 needs an FMB_MIXED plan on the matrix cores");

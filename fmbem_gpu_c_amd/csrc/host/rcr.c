@@ -12,7 +12,12 @@
  * Buckets are cut into chunks of at most RCR_CHUNK pairs.  The blocks of one
  * translation are split into units (whole blocks when the batch sits in LDS
  * and is updated in place; row groups otherwise) and assigned to the
- * workgroup's waves by longest-processing-time first. */
+ * workgroup's waves by longest-processing-time first.
+ *
+ * The FP32 / FP64 rotation blocks run in even/odd coordinates
+ * (csrc/hip/rcr_mfma.hip): degree n's (2n+1)^2 block becomes an even block
+ * of n + 1 rows and an odd block of n rows, about half the products, in one
+ * unit per degree.  The split-BF16 rotations keep the standard blocks. */
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -83,14 +88,18 @@ static int group_tiles(int nt, int g, int mt, int split) {
 static int pad_rows(int s) { return 16 * tiles(s); }
 static int pad_cols(int s) { return 4 * ksteps(s); }
 
-/* offsets of the padded rotation blocks n < np (np + 1 values) */
+/* values of one padded even/odd rotation block of s rows (0 for s = 0) */
+static int rot_block(int s) { return s ? pad_rows(s) * pad_cols(s) : 0; }
+
+/* offsets of the padded rotation blocks of degrees n < np (np + 1 values):
+ * degree n's even block (n + 1 rows), then its odd block (n rows) */
 static void rot_offsets(int np, int *hoff) {
   hoff[0] = 0;
-  for (int n = 0; n < np; n++) hoff[n + 1] = hoff[n] + pad_rows(2 * n + 1) * pad_cols(2 * n + 1);
+  for (int n = 0; n < np; n++) hoff[n + 1] = hoff[n] + rot_block(n + 1) + rot_block(n);
 }
 
-/* offsets of the split-BF16 rotation blocks n < np (np + 1 values): rows
- * padded to 16, K to 32 */
+/* offsets of the split-BF16 (standard) rotation blocks n < np (np + 1
+ * values): rows padded to 16, K to 32 */
 static void rot_offsets_bf(int np, int *hb) {
   hb[0] = 0;
   for (int n = 0; n < np; n++) hb[n + 1] = hb[n] + pad_rows(2 * n + 1) * (((2 * n + 1) + 31) & ~31);
@@ -110,14 +119,22 @@ static void coax_offsets(int pin, int pout, int *coff) {
     coff[am + 1] = coff[am] + pad_rows(pout - am) * pad_cols(pin - am);
 }
 
-/* rotation blocks n < np: code n*16 + g */
-static int rot_units(int np, int split, int mt, const int *hoff, unit_t *u) {
+/* rotation blocks of degrees n < np: code n*16 + g.  FP32 / FP64: degree
+ * n's even (n + 1 rows) and odd (n rows) blocks together; split-BF16: the
+ * standard block (2n + 1 rows). */
+static int rot_units(int np, int split, int mt, const int *hoff, int bf, unit_t *u) {
   int k = 0;
   for (int n = 0; n < np; n++) {
-    int s = 2 * n + 1, nt = tiles(s), ng = split ? (nt + mt - 1) / mt : 1;
+    int s = bf ? 2 * n + 1 : n + 1, nt = tiles(s), ng = split ? (nt + mt - 1) / mt : 1;
     for (int g = 0; g < ng; g++) {
       u[k].code = n * 16 + g;
       u[k].cost = group_tiles(nt, g, mt, split) * ksteps(s);
+      if (!bf) {
+        /* the odd block's tiles in this row group */
+        int to = split ? tiles(n) - g * mt : tiles(n);
+        to = to < 0 ? 0 : (split && to > mt ? mt : to);
+        u[k].cost += to * ksteps(n);
+      }
       u[k].aoff = hoff[n];
       k++;
     }
@@ -149,16 +166,16 @@ static int coax_units(int pin, int pout, int split, int mt, const int *coff,
  * the int pair {code, operator offset} at [2q, 2q + 1] of the same array.
  * The header is 3 (nw + 1) ints rounded up to even. */
 static int build_sched(int pin, int pout, int P, int mode, int mt, int nw, const int *hoff,
-                       const int *coff, int **d_out, int *len, void *stream) {
+                       int bf, const int *coff, int **d_out, int *len, void *stream) {
   int split = mode == FMBH_RCR_GWS, rc;
   if (nw < 1 || nw > RCR_WAVES_MAX)
     return fmb_set_error(FMB_ERR_RANGE, "translation schedule for %d waves", nw);
   const int hdr = (3 * (nw + 1) + 1) & ~1;
-  unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + 2 * P + pout));
+  unit_t *u = (unit_t *)malloc(sizeof(unit_t) * 16 * (pin + P + pout));
   if (!u) return fmb_set_error(FMB_ERR_NOMEM, "translation schedule");
-  int n1 = rot_units(pin, split, mt, hoff, u);
+  int n1 = rot_units(pin, split, mt, hoff, bf, u);
   int nc = coax_units(pin, pout, split, mt, coff, u + n1);
-  int n3 = rot_units(pout, split, mt, hoff, u + n1 + nc);
+  int n3 = rot_units(pout, split, mt, hoff, bf, u + n1 + nc);
   int tot = hdr + 2 * (n1 + nc + n3);
   int *h = (int *)calloc(tot, sizeof(int));
   if (!h) {
@@ -322,8 +339,10 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
           break;
         }
     tb->bf[op] = use_bf(pl, op, tb->mode[op]);
+    /* split-BF16 rotations keep the standard blocks: whole 2P - 1 rows */
+    if (tb->bf[op] && fmbh_rcr_mt_lds(tb->P, 1)) tb->mt[op] = fmbh_rcr_mt_lds(tb->P, 1);
     if ((rc = build_sched(pin, pout, tb->P, tb->mode[op], tb->mt[op], tb->nw[op],
-                          tb->bf[op] ? hboff : hoff, coff, &tb->d_sched[op],
+                          tb->bf[op] ? hboff : hoff, tb->bf[op], coff, &tb->d_sched[op],
                           &tb->sched_len[op], stream)))
       break;
     if (!d_coff) {
@@ -394,8 +413,9 @@ int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
     rc = fmbh_malloc(&o->d_Hpad, bytes);
     if (!rc) {
       pl->bytes += bytes;
-      rc = fmbh_pad_rot(o->prec, o->d_H, pl->nbeta, pl->Pmax, d_hoff, o->hpad_stride,
-                        o->d_Hpad, stream);
+      /* from the FP64 blocks (opset 0), rounded once */
+      rc = fmbh_pad_rot(o->prec, (const double *)pl->ops[0].d_H, pl->nbeta, pl->Pmax, d_hoff,
+                        o->hpad_stride, o->d_Hpad, stream);
     }
     /* split-BF16 copies for the FP32 pass */
     if (!rc && use_bf(pl, op, FMBH_RCR_LDS)) {
@@ -410,7 +430,7 @@ int fmbx_rcr_rot_setup(fmb_plan *pl, void *stream) {
         if (!rc) rc = fmbh_malloc(&o->d_Hb, hbytes);
         if (!rc) {
           pl->bytes += hbytes;
-          rc = fmbh_split_rot(o->d_Hpad, o->hpad_stride, d_hoff, pl->nbeta, pl->Pmax, d_hb,
+          rc = fmbh_split_rot((const double *)pl->ops[0].d_H, pl->nbeta, pl->Pmax, d_hb,
                               o->hb_half, o->d_Hb, stream);
         }
         if (!rc) rc = fmbh_sync(stream);
@@ -471,16 +491,19 @@ typedef struct {
 
 /* bfok: LDS shapes are also tried with split-BF16 rotations */
 static int shape_candidates(int P, int bfok, shape_t *c) {
-  int n = 0, mt = fmbh_rcr_mt_lds(P);
+  int n = 0, mt = fmbh_rcr_mt_lds(P, 0);
   static const int mts[4] = {2, 3, 4, 6};
   for (int nw = 4; nw <= 8; nw += 4) {
     for (int bf = 0; bf <= bfok; bf++)
-      for (int nt = 1; nt <= 2; nt++)
-        if (mt) {
-          shape_t v = {FMBH_RCR_LDS, nt, mt, nw, bf}, w = {FMBH_RCR_LDSP, nt, mt, nw, bf};
+      for (int nt = 1; nt <= 2; nt++) {
+        /* split-BF16 rotations keep the standard (2P - 1 row) blocks */
+        const int m = fmbh_rcr_mt_lds(P, bf);
+        if (m) {
+          shape_t v = {FMBH_RCR_LDS, nt, m, nw, bf}, w = {FMBH_RCR_LDSP, nt, m, nw, bf};
           c[n++] = v;
           c[n++] = w;
         }
+      }
     for (int nt = 1; nt <= 2; nt++)
       if (mt) {
         shape_t v = {FMBH_RCR_LDSO, nt, mt, nw, 0};
@@ -699,7 +722,7 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
         int *sched = NULL, slen = 0;
         const int bf = c[i].bf;
         if ((rc = build_sched(tb->pin, tb->pout, tb->P, c[i].mode, c[i].mt, c[i].nw,
-                              bf ? hboff : hoff, coff, &sched, &slen, stream)))
+                              bf ? hboff : hoff, bf, coff, &sched, &slen, stream)))
           break;
         if (c[i].mode == FMBH_RCR_LDS || c[i].mode == FMBH_RCR_LDSP ||
             c[i].mode == FMBH_RCR_LDSO) {
@@ -760,6 +783,20 @@ static int autotune(fmb_plan *pl, int op, void *stream) {
   return rc;
 }
 
+/* The tile-major M2L order keeps a tile's sources in its XCD's L2.  It pays
+ * where the level's FP64 source coefficients (boxes x right-hand sides x
+ * p^2 x 16 bytes) outgrow the GPU's 256 MB last-level (Infinity) cache, the
+ * default m2l_tile_min_mb; below that the offset-major order's fuller chunks
+ * win (measured: the level-6 and level-7 M2L of the 327,680-triangle sphere
+ * at 9 right-hand sides 5-9% faster tile-major; the 20,480- and
+ * 81,920-triangle spheres at one right-hand side 6-10% slower per matvec). */
+static int m2l_tiled(const fmb_plan *pl, int l) {
+  if (pl->prm.m2l_tile <= 0) return 0;
+  const size_t p2 = (size_t)pl->p[l] * pl->p[l];
+  const size_t bytes = (size_t)pl->tree->nbox[l] * pl->nrhs_max * p2 * 16;
+  return bytes > ((size_t)(pl->prm.m2l_tile_min_mb > 0 ? pl->prm.m2l_tile_min_mb : 0) << 20);
+}
+
 int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
   const fmb_tree *t = pl->tree;
   int L = t->lmax, rc = FMB_OK;
@@ -787,7 +824,8 @@ int fmbx_rcr_setup(fmb_plan *pl, void *stream) {
     for (int l = 2; l <= L && !rc; l++) {
       fmb_rcr_tab *tb = &pl->rcr[0][l];
       void *C[2] = {pl->ops[0].d_Csr[l], pl->ops[1].d_Csr[l]};
-      if (pl->prm.m2l_tile > 0)
+      tb->tiled = m2l_tiled(pl, l);
+      if (tb->tiled)
         rc = build_chunks_tiled(t, l, pl->prm.m2l_tile, tb, stream);
       else
         rc = build_chunks(t->h_off_start[l], FMB_NOFF, tb, stream);
@@ -868,7 +906,7 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
   j.hlen = tb->hlen;
-  j.xcd = kind == 0 && pl->prm.m2l_tile > 0 && j.mode == FMBH_RCR_LDS;
+  j.xcd = kind == 0 && tb->tiled && j.mode == FMBH_RCR_LDS;
   j.gws = (pl->aux && stream == pl->aux) ? (char *)o->d_gws + o->gws_slot : o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase[kind];
@@ -938,7 +976,7 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
   j.sched = tb->d_sched[op];
   j.sched_len = tb->sched_len[op];
   j.hlen = tb->hlen;
-  j.xcd = kind == 0 && pl->prm.m2l_tile > 0 && j.mode == FMBH_RCR_LDS;
+  j.xcd = kind == 0 && tb->tiled && j.mode == FMBH_RCR_LDS;
   j.gws = o->d_gws;
   j.gws_grid = o->gws_grid;
   j.phase = o->d_phase_c[kind];

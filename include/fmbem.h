@@ -204,7 +204,13 @@ typedef struct fmb_plan_params {
                                 target boxes for every offset, then the next
                                 tile), each XCD taking a contiguous range of
                                 tiles, so that a tile's sources and targets
-                                stay in its caches (default 256)            */
+                                stay in its caches, on the levels whose FP64
+                                source coefficients (boxes x nrhs_max x p^2
+                                x 16 B) exceed m2l_tile_min_mb; offset-major
+                                on the others (default 256)                 */
+  int m2l_tile_min_mb;       /* the size above which a level's M2L runs tile-
+                                major, in MiB (default 256, the GPU's last-
+                                level cache; 0: every level)               */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);

@@ -12,7 +12,8 @@ converged that is not.
   residual compares false with every tolerance and would read as
   converged);
 * a graph batch listing one plan twice gives FMB_ERR_ARG (its branches would
-  share the plan's buffers concurrently)."""
+  share the plan's buffers concurrently);
+* negative M2L work-order parameters give FMB_ERR_ARG."""
 import numpy as np
 import pytest
 import torch
@@ -91,6 +92,13 @@ def test_nan_and_inf_inputs_are_refused(small_plan):
     # the plan still solves afterwards
     x, info = pl.gmres(b, tol=1e-6)
     assert max(info.relres[:2]) <= 1e-6
+
+
+def test_negative_m2l_tile_parameters_are_refused(small_plan):
+    t, _ = small_plan
+    k = wavenumber(1000.0)
+    assert _code(F.Plan, t, k, m2l_tile=-1) == ERR_ARG
+    assert _code(F.Plan, t, k, m2l_tile_min_mb=-1) == ERR_ARG
 
 
 def test_graph_batch_refuses_a_repeated_plan(small_plan):

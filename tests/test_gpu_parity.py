@@ -472,17 +472,18 @@ def test_operators_in_lds_match_oracle():
 
 @pytest.mark.parametrize("tile", [1, 7, 64])
 def test_tile_major_m2l_order_matches(tile):
-    """M2L work in tile-major order (the pairs of `tile` consecutive target
-    boxes for every offset, each XCD taking a contiguous range) against the
-    offset-major default on the 20,480-triangle sphere at depth 6, 12 kHz:
-    the same translations in another order, so FP64 1e-13 and FP32 1e-6
-    relative, forward and transposed, twice in a row."""
+    """M2L work in tile-major order on every level (the pairs of `tile`
+    consecutive target boxes for every offset, each XCD taking a contiguous
+    range; m2l_tile_min_mb = 0 lifts the size rule) against the offset-major
+    order on the 20,480-triangle sphere at depth 6, 12 kHz: the same
+    translations in another order, so FP64 1e-13 and FP32 1e-6 relative,
+    forward and transposed, twice in a row."""
     m = F.Mesh.icosphere(5, 0.1)
     t = F.Tree(m, depth=6)
     k = wavenumber(12000.0)
     kw = dict(precision=F.FMB_MIXED, nrhs_max=3)
-    ref = F.Plan(t, k, **kw)
-    til = F.Plan(t, k, m2l_tile=tile, **kw)
+    ref = F.Plan(t, k, m2l_tile=0, **kw)
+    til = F.Plan(t, k, m2l_tile=tile, m2l_tile_min_mb=0, **kw)
     n = m.arrays()[1].shape[0]
     rng = np.random.default_rng(73)
     x = torch.tensor(rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n)), device="cuda")
