@@ -4,8 +4,9 @@ Wigner d^1, the expansion of R_n^m about a rotated frame, agreement with the
 reference's own rotation recurrence (sparseRotMatGen, translation.cu:
 1171-1237) where that recurrence is still accurate (p <= 12), the (S|R) and
 (R|R) addition theorems evaluated directly, the coaxial symmetry
-(E|F)_{n'n} = (-1)^{n+n'} (E|F)_{nn'}, and the reference invariant
-reduce(p1 -> p2) == generate(p2) (main.c:29-38)."""
+(E|F)_{n'n} = (-1)^{n+n'} (E|F)_{nn'}, the reference invariant
+reduce(p1 -> p2) == generate(p2) (main.c:29-38), and the m <-> -m symmetry
+of the rotation blocks behind the even/odd split of the translations."""
 import numpy as np
 import pytest
 
@@ -49,6 +50,39 @@ def test_wigner_d1_closed_form():
     H = np.array([[eps(mp) * (-1) ** m * eps(m) * d1[mp + 1, m + 1] for m in (-1, 0, 1)]
                   for mp in (-1, 0, 1)])
     np.testing.assert_allclose(B, H, atol=1e-15)
+
+
+@pytest.mark.parametrize("beta", [0.3, np.arccos(1 / np.sqrt(3)), np.pi / 2, 2.5])
+def test_rotation_even_odd_split(beta):
+    """The symmetries the matrix-core translations rely on: H_n^{m'm} =
+    H_n^{-m',-m} and H_n symmetric (checked to p = 60, also against the
+    reference's recurrence at p = 10).  In the coordinates e_m = (z_m +
+    z_{-m}) / 2 (e_0 = z_0), o_m = (z_m - z_{-m}) / 2, y = H_n z then splits
+    into an even block (E_{m'0} = H^{m'0}, E_{m'm} = H^{m'm} + H^{m',-m}) and
+    an odd block (O_{m'm} = H^{m'm} - H^{m',-m}), recombined as y_{+-m} =
+    e_m +- o_m: checked here against the full product on random vectors."""
+    rng = np.random.default_rng(11)
+    # (the reference's recurrence carries about 1e-11 of rounding at p = 10)
+    for H, p, tol in ((O.rot_H(60, beta), 60, 1e-13), (O.rot_H_ref(10, beta), 10, 1e-10)):
+        for n, B in enumerate(_blocks(H, p)):
+            np.testing.assert_allclose(B, B[::-1, ::-1], atol=tol)
+            np.testing.assert_allclose(B, B.T, atol=tol)
+    for n, B in enumerate(_blocks(O.rot_H(40, beta), 40)):
+        h = lambda mp, m: B[mp + n, m + n]
+        E = np.array([[h(i, k) + h(i, -k) if k else h(i, 0) for k in range(n + 1)]
+                      for i in range(n + 1)])
+        Od = np.array([[h(i, k) - h(i, -k) for k in range(1, n + 1)] for i in range(1, n + 1)])
+        z = rng.standard_normal(2 * n + 1) + 1j * rng.standard_normal(2 * n + 1)
+        zp, zm = z[n:], z[n::-1]          # z_{m}, z_{-m}, m = 0..n
+        e = (zp + zm) / 2
+        e[0] = z[n]
+        ye = E @ e
+        yo = Od @ ((zp - zm) / 2)[1:] if n else np.zeros(0)
+        y = np.empty(2 * n + 1, complex)
+        y[n] = ye[0]
+        y[n + 1:] = ye[1:] + yo
+        y[:n] = (ye[1:] - yo)[::-1]
+        np.testing.assert_allclose(y, B @ z, atol=1e-12 * max(1.0, np.abs(z).max()))
 
 
 @pytest.mark.parametrize("beta", [0.4, np.arccos(1 / np.sqrt(3)), 2.6])
