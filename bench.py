@@ -211,6 +211,21 @@ def run_sphere(a, w, torch, F, log):
     err_mv = float(torch.linalg.norm(y32 - y64) / torch.linalg.norm(y64))
     del y64, y32
 
+    # the plain FP64 path: an FP64 plan, one right-hand side per pass (neither
+    # of the method's precision and batching changes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p64 = F.Plan(tree, k, precision=F.FMB_FP64, nrhs_max=1, **plan_kwargs(a))
+    torch.cuda.synchronize()
+    plain_setup = time.perf_counter() - t0
+    x1, y1 = x[:1].contiguous(), torch.empty_like(x[:1])
+    dt1, _, _ = timed_matvecs(torch, p64, x1, y1, F.FMB_FP64, a.steps, a.warmup, flops)
+    plain = {"matvecs_per_s": a.steps / dt1, "ms_per_matvec": 1e3 * dt1 / a.steps,
+             "setup_s": plain_setup,
+             "path": "FP64 plan, one right-hand side per pass, same tree and orders"}
+    del x1, y1
+    log("plain fp64 %.3f ms/matvec" % plain["ms_per_matvec"])
+
     # dominant kernel: the M2L launch of the level with the largest time
     lbig = max(m2l32, key=lambda l: m2l32[l])
     ach32 = flops[lbig] / (m2l32[lbig] * 1e-3) / 1e12
@@ -234,6 +249,20 @@ def run_sphere(a, w, torch, F, log):
                                relres=max(info.relres[r] for r in range(nrhs)), x=xs)
             log("solve %s %.2fs iters %d relres %.2e" % (name, sec, info.iters,
                                                         solve[name]["relres"]))
+        # the plain FP64 path solves the right-hand sides one after another
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters, rel = [], 0.0
+        for r in range(nrhs):
+            xr, info = p64.gmres(b[r:r + 1].contiguous(), restart=30, tol=1e-6, maxit=3000,
+                                 check=False)
+            iters.append(info.iters)
+            rel = max(rel, info.relres[0])
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        solve["plain_fp64"] = dict(seconds=sec, seconds_incl_setup=sec + plain_setup,
+                                   iters_per_rhs=iters, relres=rel)
+        log("solve plain fp64 %.2fs iters %s" % (sec, iters))
         u64, u32 = solve["fp64"]["x"].cpu().numpy(), solve["mixed"]["x"].cpu().numpy()
         err_mixed = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
         if a.inner_eps > 0:
@@ -275,6 +304,7 @@ def run_sphere(a, w, torch, F, log):
         for s in solve.values():
             s.pop("x", None)
 
+    del p64
     cpu = None
     if not a.no_cpu:
         ts, cores, csetup = oracle_matvec_seconds(V, Fc, k, L, plan.orders())
@@ -308,6 +338,10 @@ def run_sphere(a, w, torch, F, log):
                    "translations": "split-bf16 rotations" if a.split_bf16 else "fp32"},
         "fp64_matvecs_per_s": mv64,
         "speedup_mixed_vs_fp64_matvec": mv32 / mv64,
+        # the same-batch ratio above isolates precision; against the plain
+        # FP64 path (one right-hand side per pass) batching counts too
+        "plain_fp64": plain,
+        "speedup_mixed_batched_vs_plain_fp64_matvec": mv32 / plain["matvecs_per_s"],
         "matvec_rel_err_mixed_vs_fp64": err_mv,
         "setup_s": setup_s,
         "setup_warm_s": setup_warm_s,
@@ -347,6 +381,13 @@ def run_sphere(a, w, torch, F, log):
             s["speedup_refined_vs_fp64_incl_warm_inner_setup"] = (
                 f64["seconds"] / (r["seconds"] + r["inner_setup_warm_s"]))
             s["surface_pressure_rel_err_refined_vs_fp64"] = err_ref
+        # against the plain FP64 path (one right-hand side at a time)
+        pl = solve["plain_fp64"]["seconds"]
+        s["speedup_mixed_batched_vs_plain_fp64"] = pl / solve["mixed"]["seconds"]
+        if "refined" in solve:
+            s["speedup_refined_batched_vs_plain_fp64"] = pl / solve["refined"]["seconds"]
+            s["speedup_refined_batched_vs_plain_fp64_incl_warm_inner_setup"] = (
+                pl / (solve["refined"]["seconds"] + solve["refined"]["inner_setup_warm_s"]))
         s["surface_pressure_rel_err_fp64_vs_mie"] = err_mie
     return out
 
