@@ -16,8 +16,10 @@ explicit orders and are compared with the oracle's FMM (translation.cu
   precisions, order 62 likewise, and order 44 in FP64 global-workspace and
   FP32 LDS mode.
 
-Also the device coaxial generator on the reference's own truncation check
-(main.c:27-38: (S|R), k = 13.1, t = 2.3, order 5 reduced to 2)."""
+The smallest orders (1 to 5, where the even/odd rotation blocks degenerate)
+run the same comparison.  Also the device coaxial generator on the
+reference's own truncation check (main.c:27-38: (S|R), k = 13.1, t = 2.3,
+order 5 reduced to 2)."""
 import numpy as np
 import pytest
 import torch
@@ -69,6 +71,38 @@ def test_global_workspace_translations_match_oracle(sphere4, orders, modes64, mo
         e32 = np.linalg.norm(y32 - want) / np.linalg.norm(want)
         print("orders %s parts %d: FP64 %.2e, FP32 %.2e vs oracle" % (orders, parts, e64, e32),
               flush=True)
+        assert e64 < 1e-9, (parts, e64)
+        assert e32 < 1e-5, (parts, e32)
+
+
+@pytest.mark.parametrize("orders,ldso", [([5, 3, 2, 1], 0), ([3, 3, 1, 1], 0), ([5, 3, 2, 1], 1)])
+def test_low_orders_match_oracle(sphere4, orders, ldso):
+    """The even/odd rotation blocks at their smallest: order 1 (degree 0
+    only, no odd block), source and target orders differing by one or two
+    (M2M and L2L between levels), equal orders; LDS mode, and with ldso = 1
+    the operators-in-LDS mode.  Far field and full operator against the
+    oracle on the same tree and orders, forward; FP64 1e-9, FP32 1e-5."""
+    m, V, Fc, t = sphere4
+    k = wavenumber(2000.0)
+    plev = dict(zip(range(2, 6), orders))
+    pl = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=3, p_rule=F.PRULE_EXPLICIT,
+                p_levels=[0, 0] + orders, ldso=ldso)
+    assert pl.orders()[2:6].tolist() == orders
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(depth=5, plev=plev)
+    rng = np.random.default_rng(62)
+    n = Fc.shape[0]
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    xt = torch.tensor(x, device="cuda")
+    full = ob.fmm_matvec(x)
+    far = full - ob.near_matvec(x)
+    for parts, want in ((2, far), (3, full)):
+        y64 = pl.matvec_parts(xt, parts, prec=F.FMB_FP64).cpu().numpy()
+        y32 = pl.matvec_parts(xt, parts, prec=F.FMB_MIXED).cpu().numpy()
+        e64 = np.linalg.norm(y64 - want) / np.linalg.norm(want)
+        e32 = np.linalg.norm(y32 - want) / np.linalg.norm(want)
+        print("orders %s ldso %d parts %d: FP64 %.2e, FP32 %.2e vs oracle"
+              % (orders, ldso, parts, e64, e32), flush=True)
         assert e64 < 1e-9, (parts, e64)
         assert e32 < 1e-5, (parts, e32)
 
