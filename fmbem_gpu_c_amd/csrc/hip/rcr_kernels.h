@@ -93,8 +93,11 @@ template <> struct Mfma<double> {
 // them stay scalar.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// reals per X row: 16 NT columns and kRowPad more for the LDS banks; the
+// kernels' row stride and every host-side size (x_bytes) use this one value
+constexpr int kRowPad = 2;
 template <int NT> struct Row {
-  static constexpr int stride = 16 * NT + 2;  // reals per X row
+  static constexpr int stride = 16 * NT + kRowPad;
 };
 
 // K-steps per operand register set
@@ -819,7 +822,7 @@ inline size_t small_lds(int pm, size_t tsz, int sched_len) {
 
 
 inline size_t x_bytes(int pm, int nt, size_t tsz) {
-  return (size_t)pm * pm * (16 * nt + 2) * tsz;
+  return (size_t)pm * pm * (16 * nt + kRowPad) * tsz;
 }
 
 // LDSO mode: batch, schedule, then (16-byte aligned) hlen rotation
