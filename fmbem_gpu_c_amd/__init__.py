@@ -21,6 +21,7 @@ PRULE_TRUNCNUM, PRULE_EXCESS, PRULE_EXPLICIT = 0, 1, 2
 RULE_COLLAPSED, RULE_GAUSS7 = 0, 1
 COAX_RR, COAX_SR = 0, 1
 RCR_AUTO, RCR_VALU = 0, 1
+RCR_MODE_RULE, RCR_MODE_LDS, RCR_MODE_GWS, RCR_MODE_LDSP, RCR_MODE_LDSO = 0, 1, 2, 3, 4
 
 
 class FmbError(RuntimeError):
@@ -56,6 +57,7 @@ class PlanParams(ctypes.Structure):
         ("ldso", ctypes.c_int),
         ("m2l_tile", ctypes.c_int),
         ("m2l_tile_min_mb", ctypes.c_int),
+        ("rcr_mode", ctypes.c_int),
     ]
 
 

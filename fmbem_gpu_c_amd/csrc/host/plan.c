@@ -503,6 +503,8 @@ int fmb_plan_create(const fmb_tree *t, const fmb_plan_params *prm,
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_path %d", prm->rcr_path);
   if (prm->waves != 0 && prm->waves != 4 && prm->waves != 8)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: waves %d (0, 4 or 8)", prm->waves);
+  if (prm->rcr_mode < FMB_RCR_MODE_RULE || prm->rcr_mode > FMB_RCR_MODE_LDSO)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: rcr_mode %d", prm->rcr_mode);
   if (prm->m2l_tile < 0 || prm->m2l_tile_min_mb < 0)
     return fmb_set_error(FMB_ERR_ARG, "fmb_plan_create: m2l_tile %d, m2l_tile_min_mb %d (>= 0)",
                          prm->m2l_tile, prm->m2l_tile_min_mb);

@@ -328,9 +328,18 @@ static int plan_tab(fmb_plan *pl, fmb_rcr_tab *tb, int pin, int pout,
     if ((rc = fmbh_rcr_shape(tb->P, o->prec, &tb->mode[op], &tb->nt[op], &tb->mt[op])))
       break;
     tb->nw[op] = pl->prm.waves ? pl->prm.waves : 4;
+    /* a requested workspace mode: GWS always fits; LDSP wherever the rule's
+     * LDS batch fits (the same LDS) */
+    if (pl->prm.rcr_mode == FMB_RCR_MODE_GWS && tb->mode[op] != FMBH_RCR_GWS) {
+      tb->mode[op] = FMBH_RCR_GWS;
+      tb->nt[op] = 2;
+      tb->mt[op] = 4;
+    } else if (pl->prm.rcr_mode == FMB_RCR_MODE_LDSP && tb->mode[op] == FMBH_RCR_LDS) {
+      tb->mode[op] = FMBH_RCR_LDSP;
+    }
     /* operators in LDS where batch and operators fit (the schedule bound of
      * fmbh_rcr_shape) */
-    if (pl->prm.ldso && tb->mode[op] != FMBH_RCR_GWS)
+    if ((pl->prm.ldso || pl->prm.rcr_mode == FMB_RCR_MODE_LDSO) && tb->mode[op] != FMBH_RCR_GWS)
       for (int nt = 2; nt >= 1; nt--)
         if (fmbh_rcr_ldso_bytes(tb->P, o->prec, nt, 8 * tb->P + 16, tb->hlen,
                                 tb->cpad_stride) <= 160 * 1024) {

@@ -158,6 +158,13 @@ typedef struct fmb_plan fmb_plan;
 #define FMB_RCR_AUTO 0 /* matrix cores (batched per offset / octant)      */
 #define FMB_RCR_VALU 1 /* one translation per workgroup on the vector ALU */
 
+/* workspace mode of the matrix-core translations (fmb_plan_params.rcr_mode) */
+#define FMB_RCR_MODE_RULE 0 /* the tile-shape rule (or the autotuner)      */
+#define FMB_RCR_MODE_LDS 1  /* batch in LDS, one workgroup per batch       */
+#define FMB_RCR_MODE_GWS 2  /* batch in a global workspace, persistent grid */
+#define FMB_RCR_MODE_LDSP 3 /* batch in LDS, persistent grid               */
+#define FMB_RCR_MODE_LDSO 4 /* as LDSP, with the bucket's operators in LDS  */
+
 typedef struct fmb_plan_params {
   double k;                  /* wavenumber (rad/m)                          */
   double alpha_re, alpha_im; /* Burton-Miller coupling; 0,0 selects i/k     */
@@ -211,6 +218,12 @@ typedef struct fmb_plan_params {
   int m2l_tile_min_mb;       /* the size above which a level's M2L runs tile-
                                 major, in MiB (default 256, the GPU's last-
                                 level cache; 0: every level)               */
+  int rcr_mode;              /* FMB_RCR_MODE_*: the fixed tile-shape rule's
+                                workspace mode for every translation where
+                                its batch (and for LDSO its operators) fit
+                                in LDS, else the rule's own; GWS always
+                                fits.  For tests and tuning studies (0: the
+                                rule; the autotuner times every mode) */
 } fmb_plan_params;
 
 void fmb_plan_params_default(fmb_plan_params *prm);

@@ -13,7 +13,8 @@ converged that is not.
   converged);
 * a graph batch listing one plan twice gives FMB_ERR_ARG (its branches would
   share the plan's buffers concurrently);
-* negative M2L work-order parameters give FMB_ERR_ARG."""
+* negative M2L work-order parameters and an unknown translation mode give
+  FMB_ERR_ARG."""
 import numpy as np
 import pytest
 import torch
@@ -94,11 +95,12 @@ def test_nan_and_inf_inputs_are_refused(small_plan):
     assert max(info.relres[:2]) <= 1e-6
 
 
-def test_negative_m2l_tile_parameters_are_refused(small_plan):
+def test_bad_translation_parameters_are_refused(small_plan):
     t, _ = small_plan
     k = wavenumber(1000.0)
     assert _code(F.Plan, t, k, m2l_tile=-1) == ERR_ARG
     assert _code(F.Plan, t, k, m2l_tile_min_mb=-1) == ERR_ARG
+    assert _code(F.Plan, t, k, rcr_mode=5) == ERR_ARG
 
 
 def test_graph_batch_refuses_a_repeated_plan(small_plan):

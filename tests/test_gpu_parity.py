@@ -243,7 +243,7 @@ def test_autotuned_shapes_give_the_same_operator():
         for kind in range(3):
             for l in range(2 if kind == 0 else 3, 6):
                 mode, nt, mt, nw, ms = tuned.tile_shape(kind, l, prec)
-                assert mode in (1, 2, 3) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
+                assert mode in (1, 2, 3, 4) and nt in (1, 2) and mt in (2, 3, 4, 6) and ms > 0
                 assert nw in (4, 8)
                 assert ref.tile_shape(kind, l, prec)[3:] == (4, 0.0)
     again = F.Plan(t, k, autotune=1, **kw)

@@ -17,7 +17,8 @@ explicit orders and are compared with the oracle's FMM (translation.cu
   FP32 LDS mode.
 
 The smallest orders (1 to 5, where the even/odd rotation blocks degenerate)
-run the same comparison.  Also the device coaxial generator on the
+and each workspace mode requested for every translation (rcr_mode) run the
+same comparison.  Also the device coaxial generator on the
 reference's own truncation check (main.c:27-38: (S|R), k = 13.1, t = 2.3,
 order 5 reduced to 2)."""
 import numpy as np
@@ -103,6 +104,45 @@ def test_low_orders_match_oracle(sphere4, orders, ldso):
         e32 = np.linalg.norm(y32 - want) / np.linalg.norm(want)
         print("orders %s ldso %d parts %d: FP64 %.2e, FP32 %.2e vs oracle"
               % (orders, ldso, parts, e64, e32), flush=True)
+        assert e64 < 1e-9, (parts, e64)
+        assert e32 < 1e-5, (parts, e32)
+
+
+@pytest.mark.parametrize("mode", [F.RCR_MODE_LDS, F.RCR_MODE_GWS, F.RCR_MODE_LDSP,
+                                  F.RCR_MODE_LDSO])
+def test_every_translation_mode_matches_oracle(sphere4, mode):
+    """Each workspace mode of the matrix-core translations requested for
+    every translation (rcr_mode; LDSO where batch and operators fit in LDS,
+    the rule's mode elsewhere) at orders [20, 14, 10, 7], 4 kHz, with the
+    mode asserted per level: far field and full operator against the oracle,
+    forward.  FP64 1e-9, FP32 1e-5."""
+    m, V, Fc, t = sphere4
+    k = wavenumber(4000.0)
+    orders = [20, 14, 10, 7]
+    plev = dict(zip(range(2, 6), orders))
+    pl = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=2, p_rule=F.PRULE_EXPLICIT,
+                p_levels=[0, 0] + orders, rcr_mode=mode)
+    for prec in (F.FMB_FP64, F.FMB_MIXED):
+        for kind in range(3):
+            for l in range(2 if kind == 0 else 3, 6):
+                got = pl.tile_shape(kind, l, prec)[0]
+                assert got == mode or (mode == F.RCR_MODE_LDSO and got == F.RCR_MODE_LDS), \
+                    (prec, kind, l, got)
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(depth=5, plev=plev)
+    rng = np.random.default_rng(63)
+    n = Fc.shape[0]
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    xt = torch.tensor(x, device="cuda")
+    full = ob.fmm_matvec(x)
+    far = full - ob.near_matvec(x)
+    for parts, want in ((2, far), (3, full)):
+        y64 = pl.matvec_parts(xt, parts, prec=F.FMB_FP64).cpu().numpy()
+        y32 = pl.matvec_parts(xt, parts, prec=F.FMB_MIXED).cpu().numpy()
+        e64 = np.linalg.norm(y64 - want) / np.linalg.norm(want)
+        e32 = np.linalg.norm(y32 - want) / np.linalg.norm(want)
+        print("mode %d parts %d: FP64 %.2e, FP32 %.2e vs oracle" % (mode, parts, e64, e32),
+              flush=True)
         assert e64 < 1e-9, (parts, e64)
         assert e32 < 1e-5, (parts, e32)
 
