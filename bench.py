@@ -234,6 +234,7 @@ def run_sphere(a, w, torch, F, log):
     traffic = traffic_of(a.workload, lbig, shape32)
 
     solve = {}
+    field = None
     err_mie = err_mixed = err_ref = None
     if not a.no_solve:
         for name, prec in (("fp64", F.FMB_FP64), ("mixed", F.FMB_MIXED)):
@@ -301,6 +302,34 @@ def run_sphere(a, w, torch, F, log):
         ct = (c @ dirs[0]) / np.linalg.norm(c, axis=1)
         mie = O.mie_rigid(k * w["radius"], ct, nterms=int(k * w["radius"]) + 40)
         err_mie = float(np.linalg.norm(u64[0] - mie) / np.linalg.norm(mie))
+        if a.field_points > 0:
+            # total pressure of the first right-hand side's FP64 solution on a
+            # sphere of two radii: through the tree and by the direct sum,
+            # each against the exterior Mie series
+            fpts = 2.0 * w["radius"] * fibonacci_dirs(a.field_points)
+            x0 = solve["fp64"]["x"][0:1].contiguous()
+            fres = {}
+            for meth in ("direct", "fmm"):
+                plan.field(x0, fpts[:64], dirs[0:1], method=meth)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                pf = plan.field(x0, fpts, dirs[0:1], method=meth)
+                torch.cuda.synchronize()
+                fres[meth] = (time.perf_counter() - t0, pf.cpu().numpy()[0])
+            mief = O.mie_rigid_field(k, w["radius"], fpts, dirs[0])
+            field = {"points": int(a.field_points), "radius_m": 2.0 * w["radius"],
+                     "direct_s": fres["direct"][0], "fmm_s": fres["fmm"][0],
+                     "speedup_fmm_vs_direct": fres["direct"][0] / fres["fmm"][0],
+                     "rel_diff_fmm_vs_direct": float(
+                         np.linalg.norm(fres["fmm"][1] - fres["direct"][1]) /
+                         np.linalg.norm(fres["direct"][1])),
+                     "rel_err_fmm_vs_mie": float(
+                         np.linalg.norm(fres["fmm"][1] - mief) / np.linalg.norm(mief)),
+                     "rel_err_direct_vs_mie": float(
+                         np.linalg.norm(fres["direct"][1] - mief) / np.linalg.norm(mief))}
+            log("field %d points: direct %.3fs, through the tree %.3fs; vs Mie %.2e / %.2e" % (
+                a.field_points, field["direct_s"], field["fmm_s"],
+                field["rel_err_direct_vs_mie"], field["rel_err_fmm_vs_mie"]))
         for s in solve.values():
             s.pop("x", None)
 
@@ -389,6 +418,8 @@ def run_sphere(a, w, torch, F, log):
             s["speedup_refined_batched_vs_plain_fp64_incl_warm_inner_setup"] = (
                 pl / (solve["refined"]["seconds"] + solve["refined"]["inner_setup_warm_s"]))
         s["surface_pressure_rel_err_fp64_vs_mie"] = err_mie
+    if field:
+        out["field_points"] = field
     return out
 
 
@@ -550,6 +581,8 @@ def main():
     ap.add_argument("--workload", default="sphere-l7-10k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--field-points", type=int, default=16384,
+                    help="field points off the sphere after the solves (0: none)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="fixed tile-shape rule instead of timing the shapes at plan creation")

@@ -118,6 +118,7 @@ _SIGS = {
     "fmb_matvec_parts": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "fmb_rhs_plane_wave": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _vp, _vp]),
     "fmb_field_eval": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "fmb_field_eval_fmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "fmb_plan_retune": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _vp]),
     "fmb_matvec_transpose": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp,
                                             ctypes.c_int, _vp]),
@@ -452,10 +453,12 @@ class Plan:
                                         _dev(b), _stream()))
         return b
 
-    def field(self, x, pts, dirs=None):
+    def field(self, x, pts, dirs=None, method="direct"):
         """Pressure at field points pts (M x 3) from the surface solution x
         (complex128 CUDA tensor, [nrhs][N] or [N]); dirs: incident unit
         plane waves (nrhs x 3) added to the scattered field, or None.
+        method: "direct" (all panels, fmb_field_eval) or "fmm" (multipole
+        expansions of the well-separated boxes, fmb_field_eval_fmm).
         Returns a [nrhs][M] complex128 CUDA tensor."""
         import torch
         nrhs = _vecs(x, self.n, "x")
@@ -466,10 +469,12 @@ class Plan:
             if d.shape[0] != nrhs:
                 raise FmbError(-1, "one incident direction per right-hand side")
         out = torch.empty((nrhs, pts.shape[0]), dtype=torch.complex128, device="cuda")
-        _check(lib().fmb_field_eval(self.h, _dev(x), nrhs, ctypes.c_void_p(pts.ctypes.data),
-                                    pts.shape[0],
-                                    ctypes.c_void_p(d.ctypes.data) if d is not None else None,
-                                    _dev(out), _stream()))
+        if method not in ("direct", "fmm"):
+            raise FmbError(-1, "field method is 'direct' or 'fmm'")
+        fn = lib().fmb_field_eval if method == "direct" else lib().fmb_field_eval_fmm
+        _check(fn(self.h, _dev(x), nrhs, ctypes.c_void_p(pts.ctypes.data), pts.shape[0],
+                  ctypes.c_void_p(d.ctypes.data) if d is not None else None,
+                  _dev(out), _stream()))
         return out
 
     def gmres(self, b, x0=None, restart=30, tol=1e-6, maxit=1000, check=True):

@@ -11,6 +11,11 @@
 //   kt[r][j] = sum_q u[r][q] int_j (dG/dn_y - sigma_j G)(y_q, y) dS_y
 //   st[r][j] = sum_q u[r][q] int_j G(y_q, y) dS_y
 // one thread per element, R right-hand sides per pass.
+//
+// Through the tree (k_field_tree): the multipole expansions M of the upward
+// pass stand in for the panels of every box well separated from the point
+// (a treecode with the separation of the M2L lists), the panels next to the
+// point are summed directly with the rule above.
 #include "fmb_device.h"
 
 namespace {
@@ -130,6 +135,254 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// ---------------------------------------------------------------- treecode
+
+constexpr int kCand = 256; // candidates of one level: <= 64 at level 2, <= 27*8 below
+
+struct TreeLevels {
+  int L;
+  int p[FMB_MAXLEV], nbox[FMB_MAXLEV];
+  const int *keys[FMB_MAXLEV];
+  const int *cstart[FMB_MAXLEV];
+  const double *scale[FMB_MAXLEV];
+  const double2 *M[FMB_MAXLEV];
+  double D, pmin[3];
+};
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Pressure at field point blockIdx.x through the tree, right-hand sides
+// r0 .. r0+kR-1 of the nrhs the upward pass ran on.  Per level l, wave 0
+// sorts the level's candidate boxes: a box whose level-l index differs by
+// 2 or more in some coordinate from the index of the level-l box holding
+// the point adds its expansion, -sum_{n<p} w_n Mt_n^m S_n^m(y - c) (Mt the
+// stored coefficients M / w_n; the P2M basis expands -dG/dn + sigma G, the
+// negative of the field kernel); a box next to it passes its children on;
+// at the leaf level its panels are summed directly.  Groups of 16, 32 or 64
+// lanes (by the level's order) take one box each; lane m of a group runs
+// column m of S_n^m up in n: the normalised Legendre recurrence
+// P_n^m = a (cos(theta) P_{n-1}^m - b P_{n-2}^m) from the table leg (a, b
+// per (n, m)) and dg[m] sin(theta)^m, and h_n by the upward recurrence
+// h_{n+1} = (2n+1)/x h_n - h_{n-1}, dominated by y_n wherever the two
+// recurrences part.  The sums are reduced in a fixed order.
+__global__ void __launch_bounds__(kT)
+    k_field_tree(TreeLevels tl, const double2 *__restrict__ leg,
+                 const double *__restrict__ dg, const double *__restrict__ pts,
+                 const double *__restrict__ tri, const double *__restrict__ geo,
+                 const int *__restrict__ perm, const int *__restrict__ bstart, int N,
+                 double k, const double2 *__restrict__ robin,
+                 const double *__restrict__ quad, int nq, const double *__restrict__ quadn,
+                 int nqn, double ratio, const double2 *__restrict__ x, int nrhs, int r0,
+                 const double *__restrict__ dirs, int npts, double2 *__restrict__ out) {
+  __shared__ int cand[2][kCand];
+  __shared__ int farl[kCand];
+  __shared__ int nearl[32];
+  __shared__ int cnt[3];
+  __shared__ double2 red[kT / 64][kR];
+  const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const double y[3] = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
+  double u[3];
+  for (int a = 0; a < 3; a++) u[a] = (y[a] - tl.pmin[a]) / tl.D;
+  const int nr = min(kR, nrhs - r0);
+  double2 acc[kR];
+#pragma unroll
+  for (int r = 0; r < kR; r++) acc[r] = make_double2(0.0, 0.0);
+  const int L = tl.L;
+  const int nb2 = min(tl.nbox[2], 64);
+  if (tid == 0) cnt[0] = nb2;
+  if (tid < nb2) cand[0][tid] = tid;
+  __syncthreads();
+  int cur = 0;
+  for (int l = 2; l <= L; l++) {
+    const int nc = cnt[0];
+    if (nc == 0) break;
+    const double sc = ldexp(1.0, l);
+    const double ty[3] = {floor(u[0] * sc), floor(u[1] * sc), floor(u[2] * sc)};
+    __syncthreads();  // cnt[0] read by every thread before wave 0 rewrites it
+    if (w == 0) {
+      int nfar = 0, nnext = 0, nnear = 0;
+      const unsigned long long lt = (1ull << lane) - 1ull;
+      for (int i0 = 0; i0 < nc; i0 += 64) {
+        const int i = i0 + lane;
+        const bool valid = i < nc;
+        const int b = valid ? cand[cur][i] : 0;
+        bool far = false;
+        if (valid) {
+          int tx, tyy, tz;
+          deinterleave3(tl.keys[l][b], l, tx, tyy, tz);
+          const double sep = fmax(fabs(tx - ty[0]), fmax(fabs(tyy - ty[1]), fabs(tz - ty[2])));
+          far = sep >= 2.0;
+        }
+        const bool nb = valid && !far;
+        const unsigned long long fm = __ballot(far);
+        if (far) {
+          const int o = nfar + __popcll(fm & lt);
+          if (o < kCand) farl[o] = b;
+        }
+        nfar += __popcll(fm);
+        if (l < L) {
+          const int c0 = nb ? tl.cstart[l][b] : 0, c1 = nb ? tl.cstart[l][b + 1] : 0;
+          const int c = c1 - c0;
+          int s = c;
+          for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(s, o, 64);
+            if (lane >= o) s += v;
+          }
+          const int base = nnext + s - c;
+          for (int j = 0; j < c; j++)
+            if (base + j < kCand) cand[cur ^ 1][base + j] = c0 + j;
+          nnext += __shfl(s, 63, 64);
+        } else {
+          const unsigned long long nm = __ballot(nb);
+          if (nb) {
+            const int o = nnear + __popcll(nm & lt);
+            if (o < 32) nearl[o] = b;
+          }
+          nnear += __popcll(nm);
+        }
+      }
+      if (lane == 0) {
+        cnt[0] = min(nnext, kCand);
+        cnt[1] = min(nfar, kCand);
+        cnt[2] = min(nnear, 32);
+      }
+    }
+    __syncthreads();
+    // expansions of the well-separated boxes
+    const int p = tl.p[l], ps = p * p;
+    const int G = p <= 16 ? 16 : (p <= 32 ? 32 : 64);
+    const int gpw = 64 / G, ng = (kT / 64) * gpw;
+    const int g = w * gpw + lane / G, lm = lane % G;
+    const double *wsc = tl.scale[l];
+    const double h = tl.D / sc;
+    for (int i = g; i < cnt[1]; i += ng) {
+      const int b = farl[i];
+      int tx, tyy, tz;
+      deinterleave3(tl.keys[l][b], l, tx, tyy, tz);
+      const double v0 = y[0] - (tl.pmin[0] + (tx + 0.5) * h);
+      const double v1 = y[1] - (tl.pmin[1] + (tyy + 0.5) * h);
+      const double v2 = y[2] - (tl.pmin[2] + (tz + 0.5) * h);
+      const double rho = sqrt(v0 * v0 + v1 * v1), r = sqrt(rho * rho + v2 * v2);
+      const double ct = v2 / r, st = rho / r, phi = atan2(v1, v0);
+      const double kr = k * r, ikr = 1.0 / kr;
+      double sx, cx;
+      sincos(kr, &sx, &cx);
+      // h_0 = -i e^{ix}/x, h_1 = -e^{ix}(x + i)/x^2
+      const double2 h0 = make_double2(sx * ikr, -cx * ikr);
+      const double2 h1 = make_double2((sx - cx * kr) * ikr * ikr, -(sx * kr + cx) * ikr * ikr);
+      const double2 *Mb = tl.M[l] + ((size_t)b * nrhs + r0) * ps;
+      for (int m = lm; m < p; m += G) {
+        double stm = 1.0, base = st;
+        for (int e = m; e > 0; e >>= 1) {
+          if (e & 1) stm *= base;
+          base *= base;
+        }
+        double P = dg[m] * stm, Pp = 0.0;
+        double sm, cm;
+        sincos(m * phi, &sm, &cm);
+        const double2 ep = make_double2(cm, sm), em = make_double2(cm, -sm);
+        double2 hc = h0, hp = make_double2(0.0, 0.0);
+        for (int n = 0; n < p; n++) {
+          if (n >= m) {
+            if (n > m) {
+              const double2 ab = leg[n * (n + 1) / 2 + m];
+              const double Pn = ab.x * (ct * P - ab.y * Pp);
+              Pp = P;
+              P = Pn;
+            }
+            const double f = wsc[n] * P;
+            const double2 t = make_double2(f * hc.x, f * hc.y);
+            const int fp = n * n + n + m, fm = n * n + n - m;
+#pragma unroll
+            for (int rr = 0; rr < kR; rr++) {
+              if (rr < nr) {
+                double2 z = cmul2(Mb[(size_t)rr * ps + fp], ep);
+                if (m > 0) {
+                  const double2 zm = cmul2(Mb[(size_t)rr * ps + fm], em);
+                  z.x += zm.x;
+                  z.y += zm.y;
+                }
+                const double2 tz = cmul2(t, z);
+                acc[rr].x -= tz.x;
+                acc[rr].y -= tz.y;
+              }
+            }
+          }
+          if (n == 0) {
+            hp = hc;
+            hc = h1;
+          } else {
+            const double c = (2.0 * n + 1.0) * ikr;
+            const double2 hn = make_double2(c * hc.x - hp.x, c * hc.y - hp.y);
+            hp = hc;
+            hc = hn;
+          }
+        }
+      }
+    }
+    if (l == L) {
+      // panels of the leaf boxes next to the point
+      const double nz[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < cnt[2]; i++) {
+        const int b = nearl[i];
+        for (int j = bstart[b] + tid; j < bstart[b + 1]; j += kT) {
+          const double *gj = geo + 8 * (size_t)j;
+          const double dx = y[0] - gj[0], dy = y[1] - gj[1], dz = y[2] - gj[2];
+          const bool close = dx * dx + dy * dy + dz * dz < ratio * ratio * gj[7] * gj[7];
+          double2 o[4];
+          panel_sum(k, y, nz, tri + 9 * (size_t)j, gj + 3, gj[6], close ? quadn : quad,
+                    close ? nqn : nq, o);
+          double2 gg = o[1];
+          if (robin) {
+            const double2 sg = robin[j];
+            gg.x -= sg.x * o[0].x - sg.y * o[0].y;
+            gg.y -= sg.x * o[0].y + sg.y * o[0].x;
+          }
+          const int c = perm[j];
+#pragma unroll
+          for (int rr = 0; rr < kR; rr++) {
+            if (rr < nr) {
+              const double2 v = x[(size_t)(r0 + rr) * N + c];
+              acc[rr].x += gg.x * v.x - gg.y * v.y;
+              acc[rr].y += gg.x * v.y + gg.y * v.x;
+            }
+          }
+        }
+      }
+    }
+    cur ^= 1;
+    __syncthreads();  // farl, nearl and cnt are rewritten by the next level
+  }
+#pragma unroll
+  for (int r = 0; r < kR; r++) {
+    double re = acc[r].x, im = acc[r].y;
+    for (int o = 32; o > 0; o >>= 1) {
+      re += __shfl_down(re, o, 64);
+      im += __shfl_down(im, o, 64);
+    }
+    if (lane == 0) red[w][r] = make_double2(re, im);
+  }
+  __syncthreads();
+  if (tid < nr) {
+    const int r = tid;
+    double2 s = make_double2(0.0, 0.0);
+    for (int i = 0; i < kT / 64; i++) {
+      s.x += red[i][r].x;
+      s.y += red[i][r].y;
+    }
+    if (dirs) {
+      const double *d = dirs + 3 * (r0 + r);
+      double sn, cs;
+      sincos(k * (d[0] * y[0] + d[1] * y[1] + d[2] * y[2]), &sn, &cs);
+      s.x += cs;
+      s.y += sn;
+    }
+    out[(size_t)(r0 + r) * npts + q] = s;
+  }
+}
+
 }  // namespace
 
 extern "C" int fmbh_field_t(const fmb_plan *pl, const fmb_c64 *u, int nrhs,
@@ -163,4 +416,76 @@ extern "C" int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
     FMB_LAUNCH_CHECK();
   }
   return FMB_OK;
+}
+
+// Treecode field of one upward pass: x, dirs and out start at the pass's
+// first right-hand side; the opset op holds its multipole expansions.
+extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int nrhs,
+                               const double *d_pts, int npts, const double *d_dirs,
+                               fmb_c64 *out, void *stream) {
+  const fmb_tree *t = pl->tree;
+  hipStream_t st = (hipStream_t)stream;
+  const fmb_opset *o = &pl->ops[op];
+  if (o->prec != FMB_FP64)
+    return fmb_set_error(FMB_ERR_ARG, "field through the tree: FP64 expansions only");
+  if (t->lmax < 2 || t->nbox[2] > 64)
+    return fmb_set_error(FMB_ERR_ARG, "field through the tree: no level-2 boxes");
+  TreeLevels tl = {};
+  tl.L = t->lmax;
+  tl.D = t->D;
+  for (int a = 0; a < 3; a++) tl.pmin[a] = t->pmin[a];
+  int pmax = 0;
+  for (int l = 2; l <= t->lmax; l++) {
+    tl.p[l] = pl->p[l];
+    tl.nbox[l] = t->nbox[l];
+    tl.keys[l] = t->d_keys[l];
+    tl.cstart[l] = l < t->lmax ? t->d_cstart[l] : NULL;
+    tl.scale[l] = pl->d_scale[l];
+    tl.M[l] = (const double2 *)o->d_M[l];
+    pmax = pl->p[l] > pmax ? pl->p[l] : pmax;
+  }
+  // normalised Legendre recurrence: (a, b) per (n, m), P_m^m / sin^m theta
+  const int ntri = pmax * (pmax + 1) / 2;
+  double *h = (double *)malloc(sizeof(double) * (2 * (size_t)ntri + pmax));
+  if (!h) return fmb_set_error(FMB_ERR_NOMEM, "field through the tree: tables");
+  for (int n = 0; n < pmax; n++)
+    for (int m = 0; m <= n; m++) {
+      double a = 0.0, b = 0.0;
+      if (n == m + 1) {
+        a = sqrt(2.0 * m + 3.0);
+      } else if (n > m + 1) {
+        a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+        b = sqrt(((double)(n - 1) * (n - 1) - (double)m * m) /
+                 (4.0 * (n - 1) * (n - 1) - 1.0));
+      }
+      h[2 * (n * (n + 1) / 2 + m)] = a;
+      h[2 * (n * (n + 1) / 2 + m) + 1] = b;
+    }
+  double d = 1.0 / sqrt(4.0 * M_PI);
+  for (int m = 0; m < pmax; m++) {
+    if (m > 0) d *= sqrt((2.0 * m + 1.0) / (2.0 * m));
+    h[2 * ntri + m] = d;
+  }
+  double *d_tab = NULL;
+  int rc = fmbh_malloc((void **)&d_tab, sizeof(double) * (2 * (size_t)ntri + pmax));
+  if (!rc && hipMemcpy(d_tab, h, sizeof(double) * (2 * (size_t)ntri + pmax),
+                       hipMemcpyHostToDevice) != hipSuccess)
+    rc = fmb_set_error(FMB_ERR_HIP, "field through the tree: table copy");
+  free(h);
+  if (rc) return rc;
+  for (int r0 = 0; r0 < nrhs && !rc; r0 += kR) {
+    hipLaunchKernelGGL(k_field_tree, dim3(npts), dim3(kT), 0, st, tl,
+                       (const double2 *)d_tab, d_tab + 2 * ntri, d_pts, t->d_tri,
+                       t->d_geo, t->d_perm, t->d_box_start, t->nelem, pl->k,
+                       (const double2 *)pl->d_robin, pl->d_quad, pl->nq, pl->d_quadn,
+                       pl->nqn, pl->prm.near_ratio, (const double2 *)x, nrhs, r0,
+                       d_dirs, npts, (double2 *)out);
+    if (hipGetLastError() != hipSuccess)
+      rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree launch");
+  }
+  // the tables are freed only after the launches that read them
+  if (hipStreamSynchronize(st) != hipSuccess && !rc)
+    rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree");
+  fmbh_free(d_tab);
+  return rc;
 }

@@ -80,6 +80,29 @@ struct OffInfo {
   double phi;
 };
 
+/* Morton keys: x is the most significant bit of each octal digit, coarse
+ * levels most significant (octree.c:132-142). */
+__device__ __forceinline__ int interleave3(int tx, int ty, int tz, int l) {
+  int key = 0;
+  for (int i = l - 1; i >= 0; i--)
+    key = key * 8 + 4 * ((tx >> i) & 1) + 2 * ((ty >> i) & 1) + ((tz >> i) & 1);
+  return key;
+}
+
+__device__ __forceinline__ void deinterleave3(int key, int l, int &tx, int &ty,
+                                              int &tz) {
+  int x = 0, y = 0, z = 0;
+  for (int i = 0; i < l; i++) {
+    int d = (key >> (3 * i)) & 7;
+    x |= ((d >> 2) & 1) << i;
+    y |= ((d >> 1) & 1) << i;
+    z |= (d & 1) << i;
+  }
+  tx = x;
+  ty = y;
+  tz = z;
+}
+
 static inline __host__ __device__ int rot_off(int n) {
   return n * (4 * n * n - 1) / 3;
 }

@@ -149,6 +149,11 @@ int fmbh_mmajor_table(int p, int **d_tab, void *stream);
 /* pressure at field points (field.hip): d_pts npts*3, d_dirs nrhs*3 or NULL */
 int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *d_pts,
                int npts, const double *d_dirs, fmb_c64 *out, void *stream);
+/* the same through the tree, from the multipole expansions that opset op
+ * (FP64) holds for the nrhs vectors of x (field.hip) */
+int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int nrhs,
+                    const double *d_pts, int npts, const double *d_dirs,
+                    fmb_c64 *out, void *stream);
 /* transposed pass pieces: (-1)^n L2P^T into L[lmax]; near^T + P2M^T into
  * ys with the given near blocks / P2M basis (NULL: the opset's own) */
 int fmbh_l2p_t(const fmb_plan *pl, int op, int nrhs, void *stream);

@@ -15,27 +15,6 @@
 
 namespace {
 
-__device__ __forceinline__ int interleave3(int tx, int ty, int tz, int l) {
-  int key = 0;
-  for (int i = l - 1; i >= 0; i--)
-    key = key * 8 + 4 * ((tx >> i) & 1) + 2 * ((ty >> i) & 1) + ((tz >> i) & 1);
-  return key;
-}
-
-__device__ __forceinline__ void deinterleave3(int key, int l, int &tx, int &ty,
-                                              int &tz) {
-  int x = 0, y = 0, z = 0;
-  for (int i = 0; i < l; i++) {
-    int d = (key >> (3 * i)) & 7;
-    x |= ((d >> 2) & 1) << i;
-    y |= ((d >> 1) & 1) << i;
-    z |= (d & 1) << i;
-  }
-  tx = x;
-  ty = y;
-  tz = z;
-}
-
 __device__ __forceinline__ int bsearch_key(const int *keys, int n, int key) {
   int lo = 0, hi = n - 1;
   while (lo <= hi) {

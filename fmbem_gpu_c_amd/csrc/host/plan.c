@@ -757,6 +757,38 @@ int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
   return rc;
 }
 
+int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
+                       const double *pts, int npts, const double *dirs,
+                       fmb_c64 *out, void *stream) {
+  if (!pl || !pl->ready || !x || nrhs < 1 || !pts || npts < 1 || !out)
+    return fmb_set_error(FMB_ERR_ARG, "fmb_field_eval_fmm: invalid arguments");
+  const fmb_tree *t = pl->tree;
+  const int L = t->lmax, N = t->nelem;
+  fmb_opset *o = &pl->ops[0];
+  double *d_pts = NULL, *d_dirs = NULL;
+  int rc = fmbh_malloc((void **)&d_pts, sizeof(double) * 3 * (size_t)npts);
+  if (!rc) rc = fmbh_h2d(d_pts, pts, sizeof(double) * 3 * (size_t)npts, stream);
+  if (!rc && dirs) rc = fmbh_malloc((void **)&d_dirs, sizeof(double) * 3 * (size_t)nrhs);
+  if (!rc && dirs) rc = fmbh_h2d(d_dirs, dirs, sizeof(double) * 3 * (size_t)nrhs, stream);
+  for (int c0 = 0; c0 < nrhs && !rc; c0 += pl->nrhs_max) {
+    const int nc = nrhs - c0 < pl->nrhs_max ? nrhs - c0 : pl->nrhs_max;
+    const fmb_c64 *xc = x + (size_t)c0 * N;
+    if ((rc = fmbh_gather(pl, 0, xc, nc, stream))) break;
+    for (int l = 2; l < L && !rc; l++)
+      rc = fmbh_memset(o->d_M[l], 0, (size_t)t->nbox[l] * nc * pl->p[l] * pl->p[l] * o->csz,
+                       stream);
+    if (!rc) rc = fmbh_p2m(pl, 0, nc, stream);
+    for (int l = L; l > 2 && !rc; l--) rc = fmbx_rcr_run(pl, 0, 1, l, nc, stream);
+    if (!rc)
+      rc = fmbh_field_tree(pl, 0, xc, nc, d_pts, npts, d_dirs ? d_dirs + 3 * c0 : NULL,
+                           out + (size_t)c0 * npts, stream);
+  }
+  if (!rc) rc = fmbh_sync(stream);
+  fmbh_free(d_pts);
+  fmbh_free(d_dirs);
+  return rc;
+}
+
 int fmb_sweep_guess(const fmb_plan *pl, double k_from, const fmb_c64 *x_from,
                     const double *dirs, int nrhs, fmb_c64 *x0, void *stream) {
   if (!pl || !pl->ready || !x_from || !x0 || nrhs < 1 || !(k_from > 0))

@@ -288,6 +288,19 @@ int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
                    const double *pts, int npts, const double *dirs, fmb_c64 *out,
                    void *stream);
 
+/* fmb_field_eval through the tree: an FP64 upward pass (P2M, M2M) on x,
+ * then per point the multipole expansions of the boxes well separated from
+ * it (their level-l box indices differ by 2 or more in some coordinate, the
+ * separation of the M2L lists; points outside the bounding cube alike) and
+ * the panels of the leaf boxes next to it by the direct rule.  The error is
+ * the truncation error of the plan's orders; the cost per point is the
+ * expansions it meets instead of all N panels.  Runs on the FP64 opset's
+ * expansion buffers (as an FP64 fmb_matvec does) in passes of at most
+ * nrhs_max vectors.  The reference has none. */
+int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
+                       const double *pts, int npts, const double *dirs,
+                       fmb_c64 *out, void *stream);
+
 /* Right-hand side p_inc + alpha dp_inc/dn for unit plane waves e^{ik d.x}
  * (dirs: nrhs*3 host doubles).  The reference has none. */
 int fmb_rhs_plane_wave(const fmb_plan *pl, const double *dirs, int nrhs,

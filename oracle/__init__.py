@@ -77,6 +77,7 @@ _SIGS = {
     "fo_bem_fmm_orders": (_ci, [_vp, _ip]),
     "fo_bem_fmm_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_bem_field": (None, [_vp, _ci, _dp, _dp, _dp, _ci, _vp, _dp, _dp]),
+    "fo_bem_field_fmm": (None, [_vp, _ci, _dp, _dp, _dp, _ci, _vp, _dp, _dp]),
     "fo_bem_near_matvec": (None, [_vp, _ci, _dp, _dp, _dp, _dp]),
     "fo_gmres": (_ci, [_vp, _ci, _ci, _cd, _ci, _dp, _dp, _dp, _dp, ctypes.POINTER(_ci),
                        ctypes.POINTER(_cd)]),
@@ -299,6 +300,38 @@ def mie_rigid(ka, cos_theta, nterms=80):
     return re + 1j * im
 
 
+def mie_rigid_field(k, a, pts, d, nterms=None):
+    """Total pressure at points pts (M x 3) off a rigid sphere of radius a at
+    the origin, for the unit plane wave e^{ik d.x}: the exterior series
+    e^{ik d.x} - sum_n (2n+1) i^n j_n'(ka)/h_n'(ka) h_n(kr) P_n(d.x/r) with
+    h_n = j_n + i y_n (scipy's spherical Bessel functions).  At r = a it is
+    mie_rigid, which sums the Wronskian form of the same series."""
+    import scipy.special as sp
+    pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))
+    d = np.asarray(d, dtype=np.float64)
+    r = np.linalg.norm(pts, axis=1)
+    ct = (pts @ d) / r
+    if nterms is None:
+        nterms = int(k * a) + 40
+    n = np.arange(nterms + 1)
+    jp = sp.spherical_jn(n, k * a, derivative=True)
+    hp = jp + 1j * sp.spherical_yn(n, k * a, derivative=True)
+    c = -(2 * n + 1) * (1j ** n) * jp / hp
+    s = np.zeros(len(r), dtype=np.complex128)
+    P0, P1 = np.ones_like(ct), ct
+    for q in range(nterms + 1):
+        if q == 0:
+            P = P0
+        elif q == 1:
+            P = P1
+        else:
+            P = ((2 * q - 1) * ct * P1 - (q - 1) * P0) / q
+            P0, P1 = P1, P
+        h = sp.spherical_jn(q, k * r) + 1j * sp.spherical_yn(q, k * r)
+        s += c[q] * h * P
+    return np.exp(1j * k * (pts @ d)) + s
+
+
 class Bem:
     """Burton-Miller collocation BEM (dense and FMM operators, GMRES)."""
 
@@ -358,10 +391,11 @@ class Bem:
     def near_matvec(self, x):
         return self._mv(lib().fo_bem_near_matvec, x)
 
-    def field(self, x, pts, dirs=None):
+    def field(self, x, pts, dirs=None, fmm=False):
         """Pressure at field points pts (M x 3) from the surface solution x
         (nrhs x N or N); dirs: incident plane-wave directions (nrhs x 3) or
-        None for the scattered field only."""
+        None for the scattered field only.  fmm: through the tree (after
+        fmm_setup) instead of the direct sum."""
         x = np.atleast_2d(np.asarray(x, dtype=np.complex128))
         nrhs = x.shape[0]
         pts = np.ascontiguousarray(pts, dtype=np.float64)
@@ -371,8 +405,9 @@ class Bem:
         d = None
         if dirs is not None:
             d = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
-        lib().fo_bem_field(self.h, nrhs, xr, xi, pts, pts.shape[0],
-                           d.ctypes.data_as(ctypes.c_void_p) if d is not None else None, ore, oim)
+        fn = lib().fo_bem_field_fmm if fmm else lib().fo_bem_field
+        fn(self.h, nrhs, xr, xi, pts, pts.shape[0],
+           d.ctypes.data_as(ctypes.c_void_p) if d is not None else None, ore, oim)
         return ore + 1j * oim
 
     def dense_row(self, i):

@@ -137,6 +137,12 @@ int fo_bem_fmm_setup(fo_bem *b, int s, int depth, const int *plev);
 int fo_bem_fmm_orders(const fo_bem *b, int *p);
 void fo_bem_fmm_matvec(const fo_bem *b, int nrhs, const double *xre,
                        const double *xim, double *yre, double *yim);
+/* Pressure at field points through the tree: the multipole expansions of
+ * the boxes well separated from each point, the panels of the leaf boxes
+ * next to it by the rule of fo_bem_field (needs fo_bem_fmm_setup). */
+void fo_bem_field_fmm(const fo_bem *b, int nrhs, const double *xre,
+                      const double *xim, const double *pts, int npts,
+                      const double *dirs, double *ore, double *oim);
 void fo_bem_near_matvec(const fo_bem *b, int nrhs, const double *xre,
                         const double *xim, double *yre, double *yim);
 

@@ -194,10 +194,53 @@ static fo_cplx entry(const fo_bem *b, int i, int j) {
   return v;
 }
 
+/* The field kernel of element j at point yq: int_j (dG/dn_y - sigma_j G) dS_y
+ * with the panel rule of the matvec (refined when |yq - c_j| < ratio *
+ * radius_j).  The same kernels and signs as the collocation entries above. */
+static fo_cplx field_panel(const fo_bem *b, int j, const double *yq) {
+  const double *t = b->tri + 9 * j;
+  const double *bary = b->bary, *wq = b->wq;
+  int nq = b->nq;
+  double d2 = 0.0, zero[3] = {0, 0, 0};
+  for (int c = 0; c < 3; c++) {
+    double d = yq[c] - b->ctr[3 * j + c];
+    d2 += d * d;
+  }
+  if (b->nqn > 0 && d2 < b->near_ratio * b->near_ratio * b->rad[j] * b->rad[j]) {
+    bary = b->baryn;
+    wq = b->wqn;
+    nq = b->nqn;
+  }
+  fo_cplx S = 0.0, K = 0.0;
+  for (int g = 0; g < nq; g++) {
+    double y[3], kr[4], ki[4];
+    for (int c = 0; c < 3; c++)
+      y[c] = bary[3 * g] * t[c] + bary[3 * g + 1] * t[3 + c] + bary[3 * g + 2] * t[6 + c];
+    fo_kernels(b->k, yq, zero, y, b->nrm + 3 * j, kr, ki);
+    double w = wq[g] * b->area[j];
+    S += w * (kr[0] + I * ki[0]);
+    K += w * (kr[1] + I * ki[1]);
+  }
+  return K - b->sig[j] * S;
+}
+
+static void field_store(const fo_bem *b, int nrhs, const fo_cplx *acc,
+                        const double *yq, int q, int npts, const double *dirs,
+                        double *ore, double *oim) {
+  for (int r = 0; r < nrhs; r++) {
+    fo_cplx v = acc[r];
+    if (dirs) {
+      const double *d = dirs + 3 * r;
+      v += cexp(I * b->k * (d[0] * yq[0] + d[1] * yq[1] + d[2] * yq[2]));
+    }
+    ore[(size_t)r * npts + q] = creal(v);
+    oim[(size_t)r * npts + q] = cimag(v);
+  }
+}
+
 /* Pressure at field points off the surface by the exterior representation
  * formula p(y) = p_inc(y) + sum_j x_j int_j (dG/dn_y - sigma_j G) dS_y, the
- * panel rule of the matvec (refined when |y - c_j| < ratio * radius_j).
- * The same kernels and signs as the collocation entries above. */
+ * panel rule of the matvec (refined when |y - c_j| < ratio * radius_j). */
 void fo_bem_field(const fo_bem *b, int nrhs, const double *xre, const double *xim,
                   const double *pts, int npts, const double *dirs, double *ore,
                   double *oim) {
@@ -206,44 +249,12 @@ void fo_bem_field(const fo_bem *b, int nrhs, const double *xre, const double *xi
   for (int q = 0; q < npts; q++) {
     const double *yq = pts + 3 * q;
     fo_cplx *acc = (fo_cplx *)calloc(nrhs, sizeof(fo_cplx));
-    double zero[3] = {0, 0, 0};
     for (int j = 0; j < N; j++) {
-      const double *t = b->tri + 9 * j;
-      const double *bary = b->bary, *wq = b->wq;
-      int nq = b->nq;
-      double d2 = 0.0;
-      for (int c = 0; c < 3; c++) {
-        double d = yq[c] - b->ctr[3 * j + c];
-        d2 += d * d;
-      }
-      if (b->nqn > 0 && d2 < b->near_ratio * b->near_ratio * b->rad[j] * b->rad[j]) {
-        bary = b->baryn;
-        wq = b->wqn;
-        nq = b->nqn;
-      }
-      fo_cplx S = 0.0, K = 0.0;
-      for (int g = 0; g < nq; g++) {
-        double y[3], kr[4], ki[4];
-        for (int c = 0; c < 3; c++)
-          y[c] = bary[3 * g] * t[c] + bary[3 * g + 1] * t[3 + c] + bary[3 * g + 2] * t[6 + c];
-        fo_kernels(b->k, yq, zero, y, b->nrm + 3 * j, kr, ki);
-        double w = wq[g] * b->area[j];
-        S += w * (kr[0] + I * ki[0]);
-        K += w * (kr[1] + I * ki[1]);
-      }
-      fo_cplx gj = K - b->sig[j] * S;
+      fo_cplx gj = field_panel(b, j, yq);
       for (int r = 0; r < nrhs; r++)
         acc[r] += gj * (xre[(size_t)r * N + j] + I * xim[(size_t)r * N + j]);
     }
-    for (int r = 0; r < nrhs; r++) {
-      fo_cplx v = acc[r];
-      if (dirs) {
-        const double *d = dirs + 3 * r;
-        v += cexp(I * b->k * (d[0] * yq[0] + d[1] * yq[1] + d[2] * yq[2]));
-      }
-      ore[(size_t)r * npts + q] = creal(v);
-      oim[(size_t)r * npts + q] = cimag(v);
-    }
+    field_store(b, nrhs, acc, yq, q, npts, dirs, ore, oim);
     free(acc);
   }
 }
@@ -577,16 +588,11 @@ static const double *Hof(const fo_bem *b, int l, double cb) {
   return NULL;
 }
 
-/* One FMM matvec: P2M, M2M, M2L + L2L, L2P + near field. */
-static void fmm_one(const fo_bem *b, const fo_cplx *x, fo_cplx *y) {
+/* The upward pass of one vector: P2M into the leaves, M2M up to level 2.
+ * M[l], l = 2..L, zeroed [nbox_l][p_l^2] arrays. */
+static void fmm_up(const fo_bem *b, const fo_cplx *x, fo_cplx **M) {
   const fo_tree *t = b->tree;
-  int L = b->L, N = b->N;
-  fo_cplx *M[16], *Lc[16];
-  for (int l = 2; l <= L; l++) {
-    size_t n = (size_t)t->nbox[l] * b->p[l] * b->p[l];
-    M[l] = (fo_cplx *)calloc(n, sizeof(fo_cplx));
-    Lc[l] = (fo_cplx *)calloc(n, sizeof(fo_cplx));
-  }
+  int L = b->L;
   int pL = b->p[L], ps = pL * pL;
   /* P2M */
 #pragma omp parallel for schedule(dynamic, 8)
@@ -624,6 +630,20 @@ static void fmm_one(const fo_bem *b, const fo_cplx *x, fo_cplx *y) {
     }
     free(cst);
   }
+}
+
+/* One FMM matvec: P2M, M2M, M2L + L2L, L2P + near field. */
+static void fmm_one(const fo_bem *b, const fo_cplx *x, fo_cplx *y) {
+  const fo_tree *t = b->tree;
+  int L = b->L, N = b->N;
+  fo_cplx *M[16], *Lc[16];
+  for (int l = 2; l <= L; l++) {
+    size_t n = (size_t)t->nbox[l] * b->p[l] * b->p[l];
+    M[l] = (fo_cplx *)calloc(n, sizeof(fo_cplx));
+    Lc[l] = (fo_cplx *)calloc(n, sizeof(fo_cplx));
+  }
+  int pL = b->p[L], ps = pL * pL;
+  fmm_up(b, x, M);
   /* M2L and L2L */
   for (int l = 2; l <= L; l++) {
     int pl = b->p[l], cs = fo_coax_size(pl);
@@ -687,6 +707,117 @@ void fo_bem_fmm_matvec(const fo_bem *b, int nrhs, const double *xre,
   }
   free(x);
   free(y);
+}
+
+/* Pressure at field points through the tree (a treecode over the FMM's
+ * multipole expansions).  Per point y, from level 2 down: a box whose
+ * index differs by 2 or more in some coordinate from the index of the
+ * level-l box that contains y (the separation of the M2L lists) adds its
+ * expansion; a box next to it passes its children to the next level; the
+ * leaf boxes next to y's leaf box add their panels by field_panel.  The
+ * P2M basis expands the matrix kernel -dG/dn_y + sigma G, the negative of
+ * the field kernel, so a box adds -sum_{n<p_l} M_n^m S_n^m(y - c_B) with
+ * S_n^m = h_n(kr) Y_n^m.  Points outside the bounding cube have indices
+ * outside [0, 2^l) and are handled alike. */
+void fo_bem_field_fmm(const fo_bem *b, int nrhs, const double *xre,
+                      const double *xim, const double *pts, int npts,
+                      const double *dirs, double *ore, double *oim) {
+  const fo_tree *t = b->tree;
+  int L = b->L, N = b->N;
+  int *cst[16] = {0};
+  for (int l = 2; l < L; l++) {
+    int nb = t->nbox[l];
+    cst[l] = (int *)malloc(sizeof(int) * (nb + 1));
+    for (int q = 0, cb = 0; q <= nb; q++) {
+      while (cb < t->nbox[l + 1] && b->par[l + 1][cb] < q) cb++;
+      cst[l][q] = cb;
+    }
+  }
+  /* M[l][box][r][p_l^2] from one upward pass per vector */
+  fo_cplx *M[16] = {0}, *Mr[16] = {0};
+  for (int l = 2; l <= L; l++) {
+    size_t ps = (size_t)b->p[l] * b->p[l];
+    M[l] = (fo_cplx *)malloc(sizeof(fo_cplx) * (size_t)t->nbox[l] * nrhs * ps);
+    Mr[l] = (fo_cplx *)malloc(sizeof(fo_cplx) * (size_t)t->nbox[l] * ps);
+  }
+  fo_cplx *x = (fo_cplx *)malloc(sizeof(fo_cplx) * N);
+  for (int r = 0; r < nrhs; r++) {
+    for (int i = 0; i < N; i++)
+      x[i] = xre[(size_t)r * N + i] + I * xim[(size_t)r * N + i];
+    for (int l = 2; l <= L; l++)
+      memset(Mr[l], 0, sizeof(fo_cplx) * (size_t)t->nbox[l] * b->p[l] * b->p[l]);
+    fmm_up(b, x, Mr);
+    for (int l = 2; l <= L; l++) {
+      size_t ps = (size_t)b->p[l] * b->p[l];
+      for (int bx = 0; bx < t->nbox[l]; bx++)
+        memcpy(M[l] + ((size_t)bx * nrhs + r) * ps, Mr[l] + bx * ps,
+               sizeof(fo_cplx) * ps);
+    }
+  }
+  int pmax = 0;
+  for (int l = 2; l <= L; l++) pmax = b->p[l] > pmax ? b->p[l] : pmax;
+  int ncap = t->nbox[2] > 216 ? t->nbox[2] : 216;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < npts; q++) {
+    const double *yq = pts + 3 * q;
+    fo_cplx *acc = (fo_cplx *)calloc(nrhs, sizeof(fo_cplx));
+    double *sre = (double *)malloc(sizeof(double) * pmax * pmax);
+    double *sim = (double *)malloc(sizeof(double) * pmax * pmax);
+    int *cand = (int *)malloc(sizeof(int) * ncap);
+    int *next = (int *)malloc(sizeof(int) * ncap);
+    int nc = t->nbox[2];
+    for (int i = 0; i < nc; i++) cand[i] = i;
+    double u[3];
+    for (int c = 0; c < 3; c++) u[c] = (yq[c] - t->pmin[c]) / t->D;
+    for (int l = 2; l <= L && nc > 0; l++) {
+      int p = b->p[l], nn = 0;
+      double sc = ldexp(1.0, l);
+      double ty[3] = {floor(u[0] * sc), floor(u[1] * sc), floor(u[2] * sc)};
+      for (int i = 0; i < nc; i++) {
+        int bx = cand[i];
+        double c[3], sep = 0.0;
+        fo_box_center(t->keys[l][bx], l, c);
+        for (int a = 0; a < 3; a++) {
+          double d = fabs(floor(c[a] * sc) - ty[a]);
+          sep = d > sep ? d : sep;
+        }
+        if (sep >= 2.0) {
+          double v[3];
+          for (int a = 0; a < 3; a++) v[a] = yq[a] - b->bc[l][3 * bx + a];
+          fo_sbasis(p - 1, b->k, v, sre, sim);
+          for (int r = 0; r < nrhs; r++) {
+            const fo_cplx *Mb = M[l] + ((size_t)bx * nrhs + r) * p * p;
+            fo_cplx s = 0.0;
+            for (int f = 0; f < p * p; f++) s += Mb[f] * (sre[f] + I * sim[f]);
+            acc[r] -= s;
+          }
+        } else if (l < L) {
+          for (int cb = cst[l][bx]; cb < cst[l][bx + 1]; cb++) next[nn++] = cb;
+        } else {
+          for (int e = t->box_start[bx]; e < t->box_end[bx]; e++) {
+            int j = t->perm[e];
+            fo_cplx gj = field_panel(b, j, yq);
+            for (int r = 0; r < nrhs; r++)
+              acc[r] += gj * (xre[(size_t)r * N + j] + I * xim[(size_t)r * N + j]);
+          }
+        }
+      }
+      for (int i = 0; i < nn; i++) cand[i] = next[i];
+      nc = nn;
+    }
+    field_store(b, nrhs, acc, yq, q, npts, dirs, ore, oim);
+    free(acc);
+    free(sre);
+    free(sim);
+    free(cand);
+    free(next);
+  }
+  for (int l = 2; l <= L; l++) {
+    free(M[l]);
+    free(Mr[l]);
+    free(cst[l]);
+  }
+  free(x);
 }
 
 void fo_bem_near_matvec(const fo_bem *b, int nrhs, const double *xre,

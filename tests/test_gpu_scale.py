@@ -9,7 +9,9 @@ minutes of CPU time):
   sums only);
 * GMRES to 1e-6 in both work precisions: the solutions agree to 1e-5, and
   the FP64 surface pressure meets the rigid-sphere Mie series to 1%
-  (0.6% is the discretisation error of flat panels at this mesh)."""
+  (0.6% is the discretisation error of flat panels at this mesh);
+* the total pressure off the surface from that solution, through the tree
+  against the direct sum (1e-8) and the exterior Mie series (1%)."""
 import numpy as np
 import pytest
 import torch
@@ -77,6 +79,21 @@ def test_full_size_solve_vs_mie(sphere7):
     print("mixed vs FP64 %.2e, FP64 vs Mie %.4f" % (err, emie), flush=True)
     assert err < 1e-5
     assert emie < 0.01
+    # total pressure off the surface (1.0 and 0.55 m from the centre) through
+    # the tree and by the direct sum, against the exterior Mie series
+    rng = np.random.default_rng(73)
+    d = rng.standard_normal((512, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pts = np.concatenate([1.0 * d[:256], 0.55 * d[256:]])
+    x0 = torch.tensor(u["fp64"][None, :], device="cuda")
+    pf = pl.field(x0, pts, [[0.0, 0.0, 1.0]], method="fmm").cpu().numpy()[0]
+    pd = pl.field(x0, pts, [[0.0, 0.0, 1.0]]).cpu().numpy()[0]
+    want = O.mie_rigid_field(k, 0.5, pts, [0.0, 0.0, 1.0])
+    etr = np.linalg.norm(pf - pd) / np.linalg.norm(pd)
+    efm = np.linalg.norm(pf - want) / np.linalg.norm(want)
+    print("field: through the tree vs direct %.2e, vs exterior Mie %.4f" % (etr, efm), flush=True)
+    assert etr < 1e-8
+    assert efm < 0.01
 
 
 def test_config2_excess_rule_mixed_vs_fp64():

@@ -138,3 +138,77 @@ def test_field_points_converge_to_exterior_mie(sphere2, sphere3, Y):
     print("field errors vs Mie", Y, errs)
     assert errs[1] < 0.02, errs
     assert errs[1] < 0.6 * errs[0], errs
+
+
+def _field_points(rng):
+    """32 points on an r = 0.5 m circle, 48 between 1.01 and 2 radii off the
+    0.1 m sphere, 16 inside it."""
+    th = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    far = np.stack([0.5 * np.cos(th), 0.5 * np.sin(th), 0.1 * np.sin(3 * th)], 1)
+    d = rng.standard_normal((64, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    near = d[:48] * rng.uniform(0.101, 0.2, 48)[:, None]
+    inside = d[48:] * rng.uniform(0.02, 0.09, 16)[:, None]
+    return np.concatenate([far, near, inside])
+
+
+@pytest.mark.parametrize("robin", [False, True])
+def test_field_through_tree_matches_direct(sphere3, robin):
+    """The treecode field (fo_bem_field_fmm) against the direct
+    representation formula (fo_bem_field): the same panels next to each
+    point, expansions for the rest, so the difference is the truncation
+    error of the orders; far points see only level-2 expansions."""
+    V, Fc = sphere3
+    k = wavenumber(1000.0)
+    rng = np.random.default_rng(5)
+    n = len(Fc)
+    sig = 1j * k * (0.01 * (1 + 0.5 * rng.uniform(-1, 1, n))
+                    + 0.005j * rng.uniform(-1, 1, n)) if robin else None
+    b = O.Bem(V, Fc, k, sigma=sig)
+    b.fmm_setup(depth=3, plev={2: 20, 3: 18})
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    pts = _field_points(rng)
+    dirs = [[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]]
+    pd = b.field(x, pts, dirs)
+    pf = b.field(x, pts, dirs, fmm=True)
+    e = np.abs(pf - pd) / np.abs(pd).max(axis=1, keepdims=True)
+    assert e[:, :32].max() < 1e-12
+    assert e.max() < 1e-5
+
+
+def test_field_through_tree_converges_with_order(sphere3):
+    """A wrong sign, index or scale in the expansion part would stall the
+    error; with the orders raised it falls by 10x per step."""
+    V, Fc = sphere3
+    k = wavenumber(1000.0)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(len(Fc)) + 1j * rng.standard_normal(len(Fc))
+    pts = _field_points(rng)[32:]
+    b = O.Bem(V, Fc, k)
+    pd = b.field(x, pts)
+    errs = []
+    for p in (12, 20, 28):
+        b = O.Bem(V, Fc, k)
+        b.fmm_setup(depth=3, plev={2: p, 3: p - 2})
+        errs.append(np.abs(b.field(x, pts, fmm=True) - pd).max() / np.abs(pd).max())
+    assert all(a > 10 * c for a, c in zip(errs, errs[1:])), errs
+
+
+@pytest.mark.parametrize("ka", [1.8, 27.5])
+def test_exterior_mie_series_meets_surface_series(ka):
+    """The exterior series (oracle.mie_rigid_field, scipy Bessel functions)
+    at r = a against the oracle's surface series (fo_mie_rigid, Wronskian
+    form): the same scattering coefficients; and far out it tends to the
+    incident wave plus an outgoing e^{ikr}/r term."""
+    k, a = ka / 0.1, 0.1
+    ct = np.linspace(-1, 1, 41)
+    st = np.sqrt(1 - ct ** 2)
+    pts = a * np.stack([st, 0 * st, ct], 1)
+    got = O.mie_rigid_field(k, a, pts, [0.0, 0.0, 1.0])
+    want = O.mie_rigid(ka, ct, nterms=int(ka) + 40)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+    r = np.array([1.0, 2.0]) * 400 * ka * ka * a
+    far = np.stack([0 * r, r * 0.6, r * 0.8], 1)
+    ps = O.mie_rigid_field(k, a, far, [0.0, 0.0, 1.0]) - np.exp(1j * k * far[:, 2])
+    f = ps * r * np.exp(-1j * k * r)
+    assert abs(f[0] - f[1]) < 0.02 * abs(f[1])
