@@ -21,6 +21,7 @@
 namespace {
 
 constexpr int kT = 256, kR = 8;
+constexpr int kFieldSlicePts = 1024; // below this many points k_field slices the panels
 
 __global__ void __launch_bounds__(kT)
     k_field(const double *__restrict__ pts, int npts, const double *__restrict__ tri,
@@ -28,16 +29,17 @@ __global__ void __launch_bounds__(kT)
             double k, const double2 *__restrict__ robin,
             const double *__restrict__ quad, int nq, const double *__restrict__ quadn,
             int nqn, double ratio, const double2 *__restrict__ x, int nrhs, int r0,
-            const double *__restrict__ dirs, double2 *__restrict__ out) {
+            const double *__restrict__ dirs, double2 *__restrict__ out,
+            double2 *__restrict__ part) {
   __shared__ double2 red[kT / 64][kR];
-  const int q = blockIdx.x;
+  const int q = blockIdx.x, nsl = gridDim.y;
   const double y[3] = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
   const double nz[3] = {0.0, 0.0, 0.0};
   const int nr = min(kR, nrhs - r0);
   double2 acc[kR];
 #pragma unroll
   for (int r = 0; r < kR; r++) acc[r] = make_double2(0.0, 0.0);
-  for (int j = threadIdx.x; j < N; j += kT) {
+  for (int j = blockIdx.y * kT + threadIdx.x; j < N; j += nsl * kT) {
     const double *gj = geo + 8 * (size_t)j;
     const double dx = y[0] - gj[0], dy = y[1] - gj[1], dz = y[2] - gj[2];
     const bool close = dx * dx + dy * dy + dz * dz < ratio * ratio * gj[7] * gj[7];
@@ -78,6 +80,10 @@ __global__ void __launch_bounds__(kT)
       s.x += red[i][r].x;
       s.y += red[i][r].y;
     }
+    if (part) {
+      part[((size_t)blockIdx.y * kR + r) * npts + q] = s;
+      return;
+    }
     if (dirs) {
       const double *d = dirs + 3 * (r0 + r);
       double sn, cs;
@@ -87,6 +93,32 @@ __global__ void __launch_bounds__(kT)
     }
     out[(size_t)(r0 + r) * npts + q] = s;
   }
+}
+
+// The panel slices of k_field summed in slice order, plus the incident
+// waves: one thread per (point, right-hand side).
+__global__ void __launch_bounds__(kT)
+    k_field_sum(const double2 *__restrict__ part, int nsl, const double *__restrict__ pts,
+                int npts, double k, int nrhs, int r0, const double *__restrict__ dirs,
+                double2 *__restrict__ out) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  const int nr = min(kR, nrhs - r0);
+  if (i >= npts * nr) return;
+  const int r = i / npts, q = i % npts;
+  double2 s = make_double2(0.0, 0.0);
+  for (int sl = 0; sl < nsl; sl++) {
+    const double2 v = part[((size_t)sl * kR + r) * npts + q];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  if (dirs) {
+    const double *d = dirs + 3 * (r0 + r);
+    double sn, cs;
+    sincos(k * (d[0] * pts[3 * q] + d[1] * pts[3 * q + 1] + d[2] * pts[3 * q + 2]), &sn, &cs);
+    s.x += cs;
+    s.y += sn;
+  }
+  out[(size_t)(r0 + r) * npts + q] = s;
 }
 
 __global__ void __launch_bounds__(kT)
@@ -138,6 +170,7 @@ __global__ void __launch_bounds__(kT)
 // ---------------------------------------------------------------- treecode
 
 constexpr int kCand = 256; // candidates of one level: <= 64 at level 2, <= 27*8 below
+constexpr int kFieldChunk = 32768; // points per pass of the level-2 sums
 
 struct TreeLevels {
   int L;
@@ -175,7 +208,8 @@ __global__ void __launch_bounds__(kT)
                  double k, const double2 *__restrict__ robin,
                  const double *__restrict__ quad, int nq, const double *__restrict__ quadn,
                  int nqn, double ratio, const double2 *__restrict__ x, int nrhs, int r0,
-                 const double *__restrict__ dirs, int npts, double2 *__restrict__ out) {
+                 const double *__restrict__ dirs, int npts, double2 *__restrict__ out,
+                 int ostride, const double2 *__restrict__ part) {
   __shared__ int cand[2][kCand];
   __shared__ int farl[kCand];
   __shared__ int nearl[32];
@@ -250,14 +284,28 @@ __global__ void __launch_bounds__(kT)
       }
     }
     __syncthreads();
-    // expansions of the well-separated boxes
+    // expansions of the well-separated boxes: level 2 from the box-major
+    // sums of k_field_l2, one box per lane of wave 0
+    if (l == 2) {
+      if (w == 0 && lane < cnt[1]) {
+        const int b = farl[lane];
+#pragma unroll
+        for (int rr = 0; rr < kR; rr++) {
+          if (rr < nr) {
+            const double2 v = part[((size_t)b * kR + rr) * npts + q];
+            acc[rr].x += v.x;
+            acc[rr].y += v.y;
+          }
+        }
+      }
+    }
     const int p = tl.p[l], ps = p * p;
     const int G = p <= 16 ? 16 : (p <= 32 ? 32 : 64);
     const int gpw = 64 / G, ng = (kT / 64) * gpw;
     const int g = w * gpw + lane / G, lm = lane % G;
     const double *wsc = tl.scale[l];
     const double h = tl.D / sc;
-    for (int i = g; i < cnt[1]; i += ng) {
+    for (int i = g; l > 2 && i < cnt[1]; i += ng) {
       const int b = farl[i];
       int tx, tyy, tz;
       deinterleave3(tl.keys[l][b], l, tx, tyy, tz);
@@ -379,7 +427,126 @@ __global__ void __launch_bounds__(kT)
       s.x += cs;
       s.y += sn;
     }
-    out[(size_t)(r0 + r) * npts + q] = s;
+    out[(size_t)(r0 + r) * ostride + q] = s;
+  }
+}
+
+// The level-2 expansions box-major: workgroup (x, b) evaluates level-2 box b
+// at kT consecutive points, one point per lane, for those points that box b
+// is well separated from (level-2 indices 2 or more apart).  Column m of the
+// box's coefficients (n = m .. p-1, orders +m and -m, every right-hand side)
+// is staged in LDS by the whole workgroup and read by every lane at one
+// address; per lane the recurrences of k_field_tree run up that column,
+// h_n restarted from n = 0.  part[(b * kR + r) * npts + q] =
+// -sum w_n Mt_n^m S_n^m(y_q - c_b) for right-hand side r0 + r; entries of
+// boxes next to a point stay unwritten (k_field_tree reads the
+// well-separated ones only).  Dynamic LDS: 2 kR p complex values.
+__global__ void __launch_bounds__(kT)
+    k_field_l2(TreeLevels tl, const double2 *__restrict__ M2,
+               const double2 *__restrict__ leg, const double *__restrict__ dg,
+               const double *__restrict__ pts, int npts, double k, int nrhs, int r0,
+               double2 *__restrict__ part) {
+  extern __shared__ double2 col[];  // [rr][sign][n - m]
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const int q = blockIdx.x * kT + tid;
+  int tx, tyy, tz;
+  deinterleave3(tl.keys[2][b], 2, tx, tyy, tz);
+  double y[3] = {0.0, 0.0, 0.0};
+  bool active = false;
+  if (q < npts) {
+    for (int a = 0; a < 3; a++) y[a] = pts[3 * q + a];
+    const double t3[3] = {(double)tx, (double)tyy, (double)tz};
+    double sep = 0.0;
+    for (int a = 0; a < 3; a++)
+      sep = fmax(sep, fabs(t3[a] - floor((y[a] - tl.pmin[a]) / tl.D * 4.0)));
+    active = sep >= 2.0;
+  }
+  if (!__syncthreads_or(active)) return;
+  const double h = tl.D * 0.25;
+  const double v0 = y[0] - (tl.pmin[0] + (tx + 0.5) * h);
+  const double v1 = y[1] - (tl.pmin[1] + (tyy + 0.5) * h);
+  const double v2 = y[2] - (tl.pmin[2] + (tz + 0.5) * h);
+  const double rho = sqrt(v0 * v0 + v1 * v1), r = sqrt(rho * rho + v2 * v2);
+  // inactive lanes run the same steps on a harmless point and store nothing
+  const double ct = active ? v2 / r : 1.0, st = active ? rho / r : 0.0;
+  const double phi = active ? atan2(v1, v0) : 0.0;
+  const double kr = active ? k * r : 1.0, ikr = 1.0 / kr;
+  double sx, cx;
+  sincos(kr, &sx, &cx);
+  const double2 h0 = make_double2(sx * ikr, -cx * ikr);
+  const double2 h1 = make_double2((sx - cx * kr) * ikr * ikr, -(sx * kr + cx) * ikr * ikr);
+  const int p = tl.p[2], ps = p * p;
+  const int nr = min(kR, nrhs - r0);
+  const double *wsc = tl.scale[2];
+  const double2 *Mb = M2 + ((size_t)b * nrhs + r0) * ps;
+  double2 acc[kR];
+#pragma unroll
+  for (int rr = 0; rr < kR; rr++) acc[rr] = make_double2(0.0, 0.0);
+  double stm = 1.0;
+  for (int m = 0; m < p; m++) {
+    const int len = p - m;
+    __syncthreads();  // the previous column is read
+    for (int i = tid; i < 2 * nr * len; i += kT) {
+      const int j = i % len, sg = (i / len) & 1, rr = i / (2 * len);
+      const int n = m + j;
+      col[(rr * 2 + sg) * p + j] = Mb[(size_t)rr * ps + n * n + n + (sg ? -m : m)];
+    }
+    __syncthreads();
+    double P = dg[m] * stm, Pp = 0.0;
+    stm *= st;
+    double sm, cm;
+    sincos(m * phi, &sm, &cm);
+    const double2 ep = make_double2(cm, sm), em = make_double2(cm, -sm);
+    double2 hc = h0, hp = make_double2(0.0, 0.0);
+    for (int n = 0; n < m; n++) {
+      if (n == 0) {
+        hp = hc;
+        hc = h1;
+      } else {
+        const double c = (2.0 * n + 1.0) * ikr;
+        const double2 hn = make_double2(c * hc.x - hp.x, c * hc.y - hp.y);
+        hp = hc;
+        hc = hn;
+      }
+    }
+    for (int n = m; n < p; n++) {
+      if (n > m) {
+        const double2 ab = leg[n * (n + 1) / 2 + m];
+        const double Pn = ab.x * (ct * P - ab.y * Pp);
+        Pp = P;
+        P = Pn;
+      }
+      const double f = wsc[n] * P;
+      const double2 t = make_double2(f * hc.x, f * hc.y);
+#pragma unroll
+      for (int rr = 0; rr < kR; rr++) {
+        if (rr < nr) {
+          double2 z = cmul2(col[(rr * 2) * p + n - m], ep);
+          if (m > 0) {
+            const double2 zm = cmul2(col[(rr * 2 + 1) * p + n - m], em);
+            z.x += zm.x;
+            z.y += zm.y;
+          }
+          const double2 tz = cmul2(t, z);
+          acc[rr].x -= tz.x;
+          acc[rr].y -= tz.y;
+        }
+      }
+      if (n == 0) {
+        hp = hc;
+        hc = h1;
+      } else {
+        const double c = (2.0 * n + 1.0) * ikr;
+        const double2 hn = make_double2(c * hc.x - hp.x, c * hc.y - hp.y);
+        hp = hc;
+        hc = hn;
+      }
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int rr = 0; rr < kR; rr++)
+      if (rr < nr) part[((size_t)b * kR + rr) * npts + q] = acc[rr];
   }
 }
 
@@ -407,15 +574,33 @@ extern "C" int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
                           fmb_c64 *out, void *stream) {
   const fmb_tree *t = pl->tree;
   hipStream_t st = (hipStream_t)stream;
-  for (int r0 = 0; r0 < nrhs; r0 += kR) {
-    hipLaunchKernelGGL(k_field, dim3(npts), dim3(kT), 0, st, d_pts, npts, t->d_tri,
+  // few points: each point's panels in slices over several workgroups, so
+  // that the launch fills the device; the slices are summed in order
+  const int nsl = npts >= kFieldSlicePts ? 1 : min(32, (2 * kFieldSlicePts + npts - 1) / npts);
+  double2 *d_part = NULL;
+  int rc = FMB_OK;
+  if (nsl > 1 && (rc = fmbh_malloc((void **)&d_part, sizeof(double2) * (size_t)nsl * kR * npts)))
+    return rc;
+  for (int r0 = 0; r0 < nrhs && !rc; r0 += kR) {
+    hipLaunchKernelGGL(k_field, dim3(npts, nsl), dim3(kT), 0, st, d_pts, npts, t->d_tri,
                        t->d_geo, t->d_perm, t->nelem, pl->k,
                        (const double2 *)pl->d_robin, pl->d_quad, pl->nq, pl->d_quadn,
                        pl->nqn, pl->prm.near_ratio, (const double2 *)x, nrhs, r0,
-                       d_dirs, (double2 *)out);
-    FMB_LAUNCH_CHECK();
+                       d_dirs, (double2 *)out, d_part);
+    if (nsl > 1) {
+      const int nr = nrhs - r0 < kR ? nrhs - r0 : kR;
+      hipLaunchKernelGGL(k_field_sum, dim3((npts * nr + kT - 1) / kT), dim3(kT), 0, st, d_part,
+                         nsl, d_pts, npts, pl->k, nrhs, r0, d_dirs, (double2 *)out);
+    }
+    if (hipGetLastError() != hipSuccess) rc = fmb_set_error(FMB_ERR_HIP, "k_field launch");
   }
-  return FMB_OK;
+  if (d_part) {
+    // the slices are freed only after the launches that read them
+    if (hipStreamSynchronize(st) != hipSuccess && !rc)
+      rc = fmb_set_error(FMB_ERR_HIP, "k_field");
+    fmbh_free(d_part);
+  }
+  return rc;
 }
 
 // Treecode field of one upward pass: x, dirs and out start at the pass's
@@ -430,6 +615,8 @@ extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int
     return fmb_set_error(FMB_ERR_ARG, "field through the tree: FP64 expansions only");
   if (t->lmax < 2 || t->nbox[2] > 64)
     return fmb_set_error(FMB_ERR_ARG, "field through the tree: no level-2 boxes");
+  if (sizeof(double2) * 2 * kR * pl->p[2] > 64 * 1024)
+    return fmb_set_error(FMB_ERR_RANGE, "field through the tree: level-2 order %d", pl->p[2]);
   TreeLevels tl = {};
   tl.L = t->lmax;
   tl.D = t->D;
@@ -473,19 +660,32 @@ extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int
     rc = fmb_set_error(FMB_ERR_HIP, "field through the tree: table copy");
   free(h);
   if (rc) return rc;
-  for (int r0 = 0; r0 < nrhs && !rc; r0 += kR) {
-    hipLaunchKernelGGL(k_field_tree, dim3(npts), dim3(kT), 0, st, tl,
-                       (const double2 *)d_tab, d_tab + 2 * ntri, d_pts, t->d_tri,
-                       t->d_geo, t->d_perm, t->d_box_start, t->nelem, pl->k,
-                       (const double2 *)pl->d_robin, pl->d_quad, pl->nq, pl->d_quadn,
-                       pl->nqn, pl->prm.near_ratio, (const double2 *)x, nrhs, r0,
-                       d_dirs, npts, (double2 *)out);
-    if (hipGetLastError() != hipSuccess)
-      rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree launch");
+  // level-2 sums of one chunk of points: [nbox_2][kR][chunk]
+  const int chunk = npts < kFieldChunk ? npts : kFieldChunk;
+  const int nb2 = t->nbox[2];
+  double2 *d_part = NULL;
+  if (!rc) rc = fmbh_malloc((void **)&d_part, sizeof(double2) * (size_t)nb2 * kR * chunk);
+  for (int q0 = 0; q0 < npts && !rc; q0 += chunk) {
+    const int nq0 = npts - q0 < chunk ? npts - q0 : chunk;
+    for (int r0 = 0; r0 < nrhs && !rc; r0 += kR) {
+      hipLaunchKernelGGL(k_field_l2, dim3((nq0 + kT - 1) / kT, nb2), dim3(kT),
+                         sizeof(double2) * 2 * kR * pl->p[2], st, tl,
+                         tl.M[2], (const double2 *)d_tab, d_tab + 2 * ntri,
+                         d_pts + 3 * (size_t)q0, nq0, pl->k, nrhs, r0, d_part);
+      hipLaunchKernelGGL(k_field_tree, dim3(nq0), dim3(kT), 0, st, tl,
+                         (const double2 *)d_tab, d_tab + 2 * ntri, d_pts + 3 * (size_t)q0,
+                         t->d_tri, t->d_geo, t->d_perm, t->d_box_start, t->nelem, pl->k,
+                         (const double2 *)pl->d_robin, pl->d_quad, pl->nq, pl->d_quadn,
+                         pl->nqn, pl->prm.near_ratio, (const double2 *)x, nrhs, r0,
+                         d_dirs, nq0, (double2 *)out + q0, npts, d_part);
+      if (hipGetLastError() != hipSuccess)
+        rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree launch");
+    }
   }
   // the tables are freed only after the launches that read them
   if (hipStreamSynchronize(st) != hipSuccess && !rc)
     rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree");
   fmbh_free(d_tab);
+  fmbh_free(d_part);
   return rc;
 }

@@ -762,6 +762,9 @@ int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
                        fmb_c64 *out, void *stream) {
   if (!pl || !pl->ready || !x || nrhs < 1 || !pts || npts < 1 || !out)
     return fmb_set_error(FMB_ERR_ARG, "fmb_field_eval_fmm: invalid arguments");
+  for (size_t i = 0; i < 3 * (size_t)npts; i++)
+    if (!isfinite(pts[i]))
+      return fmb_set_error(FMB_ERR_ARG, "fmb_field_eval_fmm: point %zu is not finite", i / 3);
   const fmb_tree *t = pl->tree;
   const int L = t->lmax, N = t->nelem;
   fmb_opset *o = &pl->ops[0];

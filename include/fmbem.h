@@ -296,7 +296,8 @@ int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
  * the truncation error of the plan's orders; the cost per point is the
  * expansions it meets instead of all N panels.  Runs on the FP64 opset's
  * expansion buffers (as an FP64 fmb_matvec does) in passes of at most
- * nrhs_max vectors.  The reference has none. */
+ * nrhs_max vectors.  Non-finite point coordinates are refused
+ * (FMB_ERR_ARG).  The reference has none. */
 int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
                        const double *pts, int npts, const double *dirs,
                        fmb_c64 *out, void *stream);
