@@ -12,10 +12,12 @@
 //   st[r][j] = sum_q u[r][q] int_j G(y_q, y) dS_y
 // one thread per element, R right-hand sides per pass.
 //
-// Through the tree (k_field_tree): the multipole expansions M of the upward
-// pass stand in for the panels of every box well separated from the point
-// (a treecode with the separation of the M2L lists), the panels next to the
-// point are summed directly with the rule above.
+// Through the tree (k_field_box, k_field_tree): the multipole expansions M
+// of the upward pass stand in for the panels of every box well separated
+// from the point (a treecode with the separation of the M2L lists), the
+// panels next to the point are summed directly with the rule above.  The
+// coarse levels, whose orders are high, run box-major over chunks of points
+// in Morton order; the finer levels per point.
 #include "fmb_device.h"
 
 namespace {
@@ -170,7 +172,10 @@ __global__ void __launch_bounds__(kT)
 // ---------------------------------------------------------------- treecode
 
 constexpr int kCand = 256; // candidates of one level: <= 64 at level 2, <= 27*8 below
-constexpr int kFieldChunk = 32768; // points per pass of the level-2 sums
+constexpr int kFieldChunk = 32768; // points per pass of the box-major sums
+constexpr int kBoxMinP = 24;        // box-major from level 2 down while orders reach this
+constexpr int kBoxMaxBoxes = 32768; // and the level has at most this many boxes
+constexpr int kBoxWG = 64;          // points per box-major workgroup
 
 struct TreeLevels {
   int L;
@@ -193,13 +198,15 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // the point adds its expansion, -sum_{n<p} w_n Mt_n^m S_n^m(y - c) (Mt the
 // stored coefficients M / w_n; the P2M basis expands -dG/dn + sigma G, the
 // negative of the field kernel); a box next to it passes its children on;
-// at the leaf level its panels are summed directly.  Groups of 16, 32 or 64
-// lanes (by the level's order) take one box each; lane m of a group runs
-// column m of S_n^m up in n: the normalised Legendre recurrence
-// P_n^m = a (cos(theta) P_{n-1}^m - b P_{n-2}^m) from the table leg (a, b
-// per (n, m)) and dg[m] sin(theta)^m, and h_n by the upward recurrence
-// h_{n+1} = (2n+1)/x h_n - h_{n-1}, dominated by y_n wherever the two
-// recurrences part.  The sums are reduced in a fixed order.
+// at the leaf level its panels are summed directly.  Levels up to lb were
+// summed box-major by k_field_box into boxsum, read back here; below lb
+// groups of 16, 32 or 64 lanes (by the level's order) take one box each,
+// and lane m of a group runs column m of S_n^m up in n: the normalised
+// Legendre recurrence P_n^m = a (cos(theta) P_{n-1}^m - b P_{n-2}^m) from
+// the table leg (a, b per (n, m)) and dg[m] sin(theta)^m, and h_n by the
+// upward recurrence h_{n+1} = (2n+1)/x h_n - h_{n-1}, dominated by y_n
+// wherever the two recurrences part.  The workgroup's sums are reduced in a
+// fixed order; the result goes to the caller's index qperm[q].
 __global__ void __launch_bounds__(kT)
     k_field_tree(TreeLevels tl, const double2 *__restrict__ leg,
                  const double *__restrict__ dg, const double *__restrict__ pts,
@@ -209,7 +216,8 @@ __global__ void __launch_bounds__(kT)
                  const double *__restrict__ quad, int nq, const double *__restrict__ quadn,
                  int nqn, double ratio, const double2 *__restrict__ x, int nrhs, int r0,
                  const double *__restrict__ dirs, int npts, double2 *__restrict__ out,
-                 int ostride, const double2 *__restrict__ part) {
+                 int ostride, const int *__restrict__ qperm, int lb,
+                 const double2 *__restrict__ boxsum) {
   __shared__ int cand[2][kCand];
   __shared__ int farl[kCand];
   __shared__ int nearl[32];
@@ -284,28 +292,15 @@ __global__ void __launch_bounds__(kT)
       }
     }
     __syncthreads();
-    // expansions of the well-separated boxes: level 2 from the box-major
-    // sums of k_field_l2, one box per lane of wave 0
-    if (l == 2) {
-      if (w == 0 && lane < cnt[1]) {
-        const int b = farl[lane];
-#pragma unroll
-        for (int rr = 0; rr < kR; rr++) {
-          if (rr < nr) {
-            const double2 v = part[((size_t)b * kR + rr) * npts + q];
-            acc[rr].x += v.x;
-            acc[rr].y += v.y;
-          }
-        }
-      }
-    }
+    // expansions of the well-separated boxes (levels up to lb: summed
+    // box-major by k_field_box into boxsum)
     const int p = tl.p[l], ps = p * p;
     const int G = p <= 16 ? 16 : (p <= 32 ? 32 : 64);
     const int gpw = 64 / G, ng = (kT / 64) * gpw;
     const int g = w * gpw + lane / G, lm = lane % G;
     const double *wsc = tl.scale[l];
     const double h = tl.D / sc;
-    for (int i = g; l > 2 && i < cnt[1]; i += ng) {
+    for (int i = g; l > lb && i < cnt[1]; i += ng) {
       const int b = farl[i];
       int tx, tyy, tz;
       deinterleave3(tl.keys[l][b], l, tx, tyy, tz);
@@ -403,6 +398,16 @@ __global__ void __launch_bounds__(kT)
     cur ^= 1;
     __syncthreads();  // farl, nearl and cnt are rewritten by the next level
   }
+  if (tid == 0) {
+#pragma unroll
+    for (int rr = 0; rr < kR; rr++) {
+      if (rr < nr) {
+        const double2 v = boxsum[(size_t)rr * npts + q];
+        acc[rr].x += v.x;
+        acc[rr].y += v.y;
+      }
+    }
+  }
 #pragma unroll
   for (int r = 0; r < kR; r++) {
     double re = acc[r].x, im = acc[r].y;
@@ -427,47 +432,55 @@ __global__ void __launch_bounds__(kT)
       s.x += cs;
       s.y += sn;
     }
-    out[(size_t)(r0 + r) * ostride + q] = s;
+    out[(size_t)(r0 + r) * ostride + qperm[q]] = s;
   }
 }
 
-// The level-2 expansions box-major: workgroup (x, b) evaluates level-2 box b
-// at kT consecutive points, one point per lane, for those points that box b
-// is well separated from (level-2 indices 2 or more apart).  Column m of the
+// The expansions of level l box-major: workgroup w evaluates box
+// w / nchunk of level l at the WG points of chunk w % nchunk (the points
+// come sorted in Morton order, so a chunk is compact), one point per lane,
+// for the points the traversal of k_field_tree meets the box at: the box is
+// well separated from the point at level l (indices 2 or more apart) and
+// its parent is next to the point at level l - 1 (no condition at level 2;
+// the parent's ancestors then lie next to the point too).  Column m of the
 // box's coefficients (n = m .. p-1, orders +m and -m, every right-hand side)
 // is staged in LDS by the whole workgroup and read by every lane at one
 // address; per lane the recurrences of k_field_tree run up that column,
-// h_n restarted from n = 0.  part[(b * kR + r) * npts + q] =
-// -sum w_n Mt_n^m S_n^m(y_q - c_b) for right-hand side r0 + r; entries of
-// boxes next to a point stay unwritten (k_field_tree reads the
-// well-separated ones only).  Dynamic LDS: 2 kR p complex values.
-__global__ void __launch_bounds__(kT)
-    k_field_l2(TreeLevels tl, const double2 *__restrict__ M2,
-               const double2 *__restrict__ leg, const double *__restrict__ dg,
-               const double *__restrict__ pts, int npts, double k, int nrhs, int r0,
-               double2 *__restrict__ part) {
+// h_n restarted from n = 0.  boxsum[r * npts + q] += -sum w_n Mt_n^m
+// S_n^m(y_q - c) for right-hand side r0 + r, by FP64 atomic adds (the order
+// of the boxes' contributions is not fixed).  Dynamic LDS: 2 p complex
+// values per right-hand side of the launch.
+template <int WG>
+__global__ void __launch_bounds__(WG)
+    k_field_box(TreeLevels tl, int l, const double2 *__restrict__ leg,
+                const double *__restrict__ dg, const double *__restrict__ pts, int npts,
+                int nchunk, double k, int nrhs, int r0, double2 *__restrict__ boxsum) {
   extern __shared__ double2 col[];  // [rr][sign][n - m]
-  const int b = blockIdx.y, tid = threadIdx.x;
-  const int q = blockIdx.x * kT + tid;
+  const int b = blockIdx.x / nchunk, tid = threadIdx.x;
+  const int q = (blockIdx.x % nchunk) * WG + tid;
   int tx, tyy, tz;
-  deinterleave3(tl.keys[2][b], 2, tx, tyy, tz);
+  deinterleave3(tl.keys[l][b], l, tx, tyy, tz);
+  const double sc = ldexp(1.0, l);
   double y[3] = {0.0, 0.0, 0.0};
   bool active = false;
   if (q < npts) {
     for (int a = 0; a < 3; a++) y[a] = pts[3 * q + a];
-    const double t3[3] = {(double)tx, (double)tyy, (double)tz};
-    double sep = 0.0;
-    for (int a = 0; a < 3; a++)
-      sep = fmax(sep, fabs(t3[a] - floor((y[a] - tl.pmin[a]) / tl.D * 4.0)));
-    active = sep >= 2.0;
+    const int t3[3] = {tx, tyy, tz};
+    double sep = 0.0, psep = 0.0;
+    for (int a = 0; a < 3; a++) {
+      const double u = (y[a] - tl.pmin[a]) / tl.D;
+      sep = fmax(sep, fabs(t3[a] - floor(u * sc)));
+      psep = fmax(psep, fabs((t3[a] >> 1) - floor(u * (0.5 * sc))));
+    }
+    active = sep >= 2.0 && (l == 2 || psep <= 1.0);
   }
   if (!__syncthreads_or(active)) return;
-  const double h = tl.D * 0.25;
+  const double h = tl.D / sc;
   const double v0 = y[0] - (tl.pmin[0] + (tx + 0.5) * h);
   const double v1 = y[1] - (tl.pmin[1] + (tyy + 0.5) * h);
   const double v2 = y[2] - (tl.pmin[2] + (tz + 0.5) * h);
   const double rho = sqrt(v0 * v0 + v1 * v1), r = sqrt(rho * rho + v2 * v2);
-  // inactive lanes run the same steps on a harmless point and store nothing
+  // inactive lanes run the same steps on a harmless point and add nothing
   const double ct = active ? v2 / r : 1.0, st = active ? rho / r : 0.0;
   const double phi = active ? atan2(v1, v0) : 0.0;
   const double kr = active ? k * r : 1.0, ikr = 1.0 / kr;
@@ -475,10 +488,10 @@ __global__ void __launch_bounds__(kT)
   sincos(kr, &sx, &cx);
   const double2 h0 = make_double2(sx * ikr, -cx * ikr);
   const double2 h1 = make_double2((sx - cx * kr) * ikr * ikr, -(sx * kr + cx) * ikr * ikr);
-  const int p = tl.p[2], ps = p * p;
+  const int p = tl.p[l], ps = p * p;
   const int nr = min(kR, nrhs - r0);
-  const double *wsc = tl.scale[2];
-  const double2 *Mb = M2 + ((size_t)b * nrhs + r0) * ps;
+  const double *wsc = tl.scale[l];
+  const double2 *Mb = tl.M[l] + ((size_t)b * nrhs + r0) * ps;
   double2 acc[kR];
 #pragma unroll
   for (int rr = 0; rr < kR; rr++) acc[rr] = make_double2(0.0, 0.0);
@@ -486,7 +499,7 @@ __global__ void __launch_bounds__(kT)
   for (int m = 0; m < p; m++) {
     const int len = p - m;
     __syncthreads();  // the previous column is read
-    for (int i = tid; i < 2 * nr * len; i += kT) {
+    for (int i = tid; i < 2 * nr * len; i += WG) {
       const int j = i % len, sg = (i / len) & 1, rr = i / (2 * len);
       const int n = m + j;
       col[(rr * 2 + sg) * p + j] = Mb[(size_t)rr * ps + n * n + n + (sg ? -m : m)];
@@ -545,8 +558,12 @@ __global__ void __launch_bounds__(kT)
   }
   if (active) {
 #pragma unroll
-    for (int rr = 0; rr < kR; rr++)
-      if (rr < nr) part[((size_t)b * kR + rr) * npts + q] = acc[rr];
+    for (int rr = 0; rr < kR; rr++) {
+      if (rr < nr) {
+        atomicAdd(&boxsum[(size_t)rr * npts + q].x, acc[rr].x);
+        atomicAdd(&boxsum[(size_t)rr * npts + q].y, acc[rr].y);
+      }
+    }
   }
 }
 
@@ -605,9 +622,11 @@ extern "C" int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
 
 // Treecode field of one upward pass: x, dirs and out start at the pass's
 // first right-hand side; the opset op holds its multipole expansions.
+// d_pts holds the points in Morton order, d_qperm[i] the caller's index of
+// sorted point i (out is in caller order).
 extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int nrhs,
-                               const double *d_pts, int npts, const double *d_dirs,
-                               fmb_c64 *out, void *stream) {
+                               const double *d_pts, const int *d_qperm, int npts,
+                               const double *d_dirs, fmb_c64 *out, void *stream) {
   const fmb_tree *t = pl->tree;
   hipStream_t st = (hipStream_t)stream;
   const fmb_opset *o = &pl->ops[op];
@@ -615,8 +634,9 @@ extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int
     return fmb_set_error(FMB_ERR_ARG, "field through the tree: FP64 expansions only");
   if (t->lmax < 2 || t->nbox[2] > 64)
     return fmb_set_error(FMB_ERR_ARG, "field through the tree: no level-2 boxes");
-  if (sizeof(double2) * 2 * kR * pl->p[2] > 64 * 1024)
-    return fmb_set_error(FMB_ERR_RANGE, "field through the tree: level-2 order %d", pl->p[2]);
+  for (int l = 2; l <= t->lmax; l++)
+    if (sizeof(double2) * 2 * kR * pl->p[l] > 64 * 1024)
+      return fmb_set_error(FMB_ERR_RANGE, "field through the tree: order %d", pl->p[l]);
   TreeLevels tl = {};
   tl.L = t->lmax;
   tl.D = t->D;
@@ -660,24 +680,35 @@ extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int
     rc = fmb_set_error(FMB_ERR_HIP, "field through the tree: table copy");
   free(h);
   if (rc) return rc;
-  // level-2 sums of one chunk of points: [nbox_2][kR][chunk]
+  // levels 2 .. lb box-major (orders of kBoxMinP and above), the rest per
+  // point; the box-major sums of one chunk of points: [kR][chunk]
+  int lb = 2;
+  while (lb < t->lmax && pl->p[lb + 1] >= kBoxMinP && t->nbox[lb + 1] <= kBoxMaxBoxes) lb++;
+  int pb = 0;
+  for (int l = 2; l <= lb; l++) pb = pl->p[l] > pb ? pl->p[l] : pb;
   const int chunk = npts < kFieldChunk ? npts : kFieldChunk;
-  const int nb2 = t->nbox[2];
-  double2 *d_part = NULL;
-  if (!rc) rc = fmbh_malloc((void **)&d_part, sizeof(double2) * (size_t)nb2 * kR * chunk);
+  double2 *d_box = NULL;
+  if (!rc) rc = fmbh_malloc((void **)&d_box, sizeof(double2) * (size_t)kR * chunk);
   for (int q0 = 0; q0 < npts && !rc; q0 += chunk) {
     const int nq0 = npts - q0 < chunk ? npts - q0 : chunk;
+    const int nchunk = (nq0 + kBoxWG - 1) / kBoxWG;
     for (int r0 = 0; r0 < nrhs && !rc; r0 += kR) {
-      hipLaunchKernelGGL(k_field_l2, dim3((nq0 + kT - 1) / kT, nb2), dim3(kT),
-                         sizeof(double2) * 2 * kR * pl->p[2], st, tl,
-                         tl.M[2], (const double2 *)d_tab, d_tab + 2 * ntri,
-                         d_pts + 3 * (size_t)q0, nq0, pl->k, nrhs, r0, d_part);
+      const int nr = nrhs - r0 < kR ? nrhs - r0 : kR;
+      if (hipMemsetAsync(d_box, 0, sizeof(double2) * (size_t)kR * nq0, st) != hipSuccess) {
+        rc = fmb_set_error(FMB_ERR_HIP, "field through the tree: clear");
+        break;
+      }
+      for (int l = 2; l <= lb; l++)
+        hipLaunchKernelGGL(k_field_box<kBoxWG>, dim3(nchunk * t->nbox[l]), dim3(kBoxWG),
+                           sizeof(double2) * 2 * nr * pb, st, tl, l, (const double2 *)d_tab,
+                           d_tab + 2 * ntri, d_pts + 3 * (size_t)q0, nq0, nchunk, pl->k,
+                           nrhs, r0, d_box);
       hipLaunchKernelGGL(k_field_tree, dim3(nq0), dim3(kT), 0, st, tl,
                          (const double2 *)d_tab, d_tab + 2 * ntri, d_pts + 3 * (size_t)q0,
                          t->d_tri, t->d_geo, t->d_perm, t->d_box_start, t->nelem, pl->k,
                          (const double2 *)pl->d_robin, pl->d_quad, pl->nq, pl->d_quadn,
                          pl->nqn, pl->prm.near_ratio, (const double2 *)x, nrhs, r0,
-                         d_dirs, nq0, (double2 *)out + q0, npts, d_part);
+                         d_dirs, nq0, (double2 *)out, npts, d_qperm + q0, lb, d_box);
       if (hipGetLastError() != hipSuccess)
         rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree launch");
     }
@@ -686,6 +717,6 @@ extern "C" int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int
   if (hipStreamSynchronize(st) != hipSuccess && !rc)
     rc = fmb_set_error(FMB_ERR_HIP, "k_field_tree");
   fmbh_free(d_tab);
-  fmbh_free(d_part);
+  fmbh_free(d_box);
   return rc;
 }

@@ -150,10 +150,11 @@ int fmbh_mmajor_table(int p, int **d_tab, void *stream);
 int fmbh_field(const fmb_plan *pl, const fmb_c64 *x, int nrhs, const double *d_pts,
                int npts, const double *d_dirs, fmb_c64 *out, void *stream);
 /* the same through the tree, from the multipole expansions that opset op
- * (FP64) holds for the nrhs vectors of x (field.hip) */
+ * (FP64) holds for the nrhs vectors of x (field.hip); d_pts in Morton
+ * order, d_qperm[i] the caller's index of point i */
 int fmbh_field_tree(const fmb_plan *pl, int op, const fmb_c64 *x, int nrhs,
-                    const double *d_pts, int npts, const double *d_dirs,
-                    fmb_c64 *out, void *stream);
+                    const double *d_pts, const int *d_qperm, int npts,
+                    const double *d_dirs, fmb_c64 *out, void *stream);
 /* transposed pass pieces: (-1)^n L2P^T into L[lmax]; near^T + P2M^T into
  * ys with the given near blocks / P2M basis (NULL: the opset's own) */
 int fmbh_l2p_t(const fmb_plan *pl, int op, int nrhs, void *stream);

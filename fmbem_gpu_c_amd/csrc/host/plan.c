@@ -757,6 +757,17 @@ int fmb_field_eval(const fmb_plan *pl, const fmb_c64 *x, int nrhs,
   return rc;
 }
 
+typedef struct field_key {
+  unsigned long long key;
+  int idx;
+} field_key;
+
+static int cmp_field_key(const void *a, const void *b) {
+  const field_key *x = (const field_key *)a, *y = (const field_key *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx - y->idx;
+}
+
 int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
                        const double *pts, int npts, const double *dirs,
                        fmb_c64 *out, void *stream) {
@@ -768,9 +779,42 @@ int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
   const fmb_tree *t = pl->tree;
   const int L = t->lmax, N = t->nelem;
   fmb_opset *o = &pl->ops[0];
+  /* points in Morton order on a grid of 1/1024 of the cube over three cube
+   * widths (points beyond clamp to its faces), so that the workgroups of
+   * the box-major levels take compact groups of points */
+  field_key *fk = (field_key *)malloc(sizeof(field_key) * (size_t)npts);
+  double *spts = (double *)malloc(sizeof(double) * 3 * (size_t)npts);
+  int *qperm = (int *)malloc(sizeof(int) * (size_t)npts);
+  if (!fk || !spts || !qperm) {
+    free(fk);
+    free(spts);
+    free(qperm);
+    return fmb_set_error(FMB_ERR_NOMEM, "fmb_field_eval_fmm: point order");
+  }
+  for (int i = 0; i < npts; i++) {
+    unsigned g[3];
+    for (int c = 0; c < 3; c++) {
+      double v = floor((pts[3 * (size_t)i + c] - t->pmin[c]) / t->D * 1024.0) + 1024.0;
+      g[c] = v < 0.0 ? 0u : (v > 3071.0 ? 3071u : (unsigned)v);
+    }
+    unsigned long long key = 0;
+    for (int b = 11; b >= 0; b--)
+      key = key * 8 + 4 * ((g[0] >> b) & 1) + 2 * ((g[1] >> b) & 1) + ((g[2] >> b) & 1);
+    fk[i].key = key;
+    fk[i].idx = i;
+  }
+  qsort(fk, (size_t)npts, sizeof(field_key), cmp_field_key);
+  for (int i = 0; i < npts; i++) {
+    qperm[i] = fk[i].idx;
+    for (int c = 0; c < 3; c++) spts[3 * (size_t)i + c] = pts[3 * (size_t)fk[i].idx + c];
+  }
+  free(fk);
   double *d_pts = NULL, *d_dirs = NULL;
+  int *d_qperm = NULL;
   int rc = fmbh_malloc((void **)&d_pts, sizeof(double) * 3 * (size_t)npts);
-  if (!rc) rc = fmbh_h2d(d_pts, pts, sizeof(double) * 3 * (size_t)npts, stream);
+  if (!rc) rc = fmbh_h2d(d_pts, spts, sizeof(double) * 3 * (size_t)npts, stream);
+  if (!rc) rc = fmbh_malloc((void **)&d_qperm, sizeof(int) * (size_t)npts);
+  if (!rc) rc = fmbh_h2d(d_qperm, qperm, sizeof(int) * (size_t)npts, stream);
   if (!rc && dirs) rc = fmbh_malloc((void **)&d_dirs, sizeof(double) * 3 * (size_t)nrhs);
   if (!rc && dirs) rc = fmbh_h2d(d_dirs, dirs, sizeof(double) * 3 * (size_t)nrhs, stream);
   for (int c0 = 0; c0 < nrhs && !rc; c0 += pl->nrhs_max) {
@@ -783,11 +827,16 @@ int fmb_field_eval_fmm(fmb_plan *pl, const fmb_c64 *x, int nrhs,
     if (!rc) rc = fmbh_p2m(pl, 0, nc, stream);
     for (int l = L; l > 2 && !rc; l--) rc = fmbx_rcr_run(pl, 0, 1, l, nc, stream);
     if (!rc)
-      rc = fmbh_field_tree(pl, 0, xc, nc, d_pts, npts, d_dirs ? d_dirs + 3 * c0 : NULL,
-                           out + (size_t)c0 * npts, stream);
+      rc = fmbh_field_tree(pl, 0, xc, nc, d_pts, d_qperm, npts,
+                           d_dirs ? d_dirs + 3 * c0 : NULL, out + (size_t)c0 * npts, stream);
   }
-  if (!rc) rc = fmbh_sync(stream);
+  /* the passes finish before their inputs are freed */
+  int rs = fmbh_sync(stream);
+  if (!rc) rc = rs;
+  free(spts);
+  free(qperm);
   fmbh_free(d_pts);
+  fmbh_free(d_qperm);
   fmbh_free(d_dirs);
   return rc;
 }

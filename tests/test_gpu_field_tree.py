@@ -119,3 +119,25 @@ def test_field_tree_config4():
     ob = O.Bem(V, Fc, k, sigma=sig)
     ob.fmm_setup(s=5)
     _compare(pl, ob, x.cpu().numpy(), _points(rng, 0.1), [[0.0, 0.0, 1.0]], 1e-4)
+
+
+def test_field_tree_chunks(sphere_l3):
+    """40,000 points (two passes of the level-2 sums, 32,768 points each at
+    most), 9 right-hand sides (two launches of 8) in upward passes of 4
+    (nrhs_max), against the oracle's treecode and the direct sum (which
+    runs unsliced above 1,024 points)."""
+    m, V, Fc = sphere_l3
+    k = wavenumber(1000.0)
+    n = Fc.shape[0]
+    rng = np.random.default_rng(35)
+    t = F.Tree(m, s=5)
+    pl = F.Plan(t, k, nrhs_max=4)
+    ob = O.Bem(V, Fc, k)
+    ob.fmm_setup(s=5)
+    d = rng.standard_normal((40000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pts = d * rng.uniform(0.05, 1.0, 40000)[:, None]
+    x = rng.standard_normal((9, n)) + 1j * rng.standard_normal((9, n))
+    dirs = rng.standard_normal((9, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    _compare(pl, ob, x, pts, dirs, 1e-4)
