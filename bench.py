@@ -304,40 +304,44 @@ def run_sphere(a, w, torch, F, log):
         err_mie = float(np.linalg.norm(u64[0] - mie) / np.linalg.norm(mie))
 
     if a.field_points > 0:
-        # total pressure on a sphere of two radii, through the tree and by
-        # the direct sum: from the first right-hand side's FP64 solution,
-        # each against the exterior Mie series, or (no solves) from a seeded
-        # random surface vector
-        fpts = 2.0 * w["radius"] * fibonacci_dirs(a.field_points)
+        # total pressure on spheres of 2 and 1.2 radii (the second within a
+        # few leaf boxes of the surface), through the tree and by the direct
+        # sum: from the first right-hand side's FP64 solution, each against
+        # the exterior Mie series, or (no solves) from a seeded random
+        # surface vector
         if solve:
             x0 = solve["fp64"]["x"][0:1].contiguous()
         else:
             rng = np.random.default_rng(11)
             x0 = torch.tensor(rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n)),
                               device="cuda")
-        fres = {}
-        for meth in ("direct", "fmm"):
-            plan.field(x0, fpts[:64], dirs[0:1], method=meth)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            pf = plan.field(x0, fpts, dirs[0:1], method=meth)
-            torch.cuda.synchronize()
-            fres[meth] = (time.perf_counter() - t0, pf.cpu().numpy()[0])
-        field = {"points": int(a.field_points), "radius_m": 2.0 * w["radius"],
-                 "direct_s": fres["direct"][0], "fmm_s": fres["fmm"][0],
-                 "speedup_fmm_vs_direct": fres["direct"][0] / fres["fmm"][0],
-                 "rel_diff_fmm_vs_direct": float(
-                     np.linalg.norm(fres["fmm"][1] - fres["direct"][1]) /
-                     np.linalg.norm(fres["direct"][1])),
+        field = {"points": int(a.field_points),
                  "source": "FP64 solution" if solve else "random surface vector"}
-        if solve:
-            import oracle as O
-            mief = O.mie_rigid_field(k, w["radius"], fpts, dirs[0])
+        for rf in (2.0, 1.2):
+            fpts = rf * w["radius"] * fibonacci_dirs(a.field_points)
+            fres = {}
             for meth in ("direct", "fmm"):
-                field["rel_err_%s_vs_mie" % meth] = float(
-                    np.linalg.norm(fres[meth][1] - mief) / np.linalg.norm(mief))
-        log("field %d points: direct %.3fs, through the tree %.3fs" % (
-            a.field_points, field["direct_s"], field["fmm_s"]))
+                plan.field(x0, fpts[:64], dirs[0:1], method=meth)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                pf = plan.field(x0, fpts, dirs[0:1], method=meth)
+                torch.cuda.synchronize()
+                fres[meth] = (time.perf_counter() - t0, pf.cpu().numpy()[0])
+            fr = {"radius_m": rf * w["radius"], "direct_s": fres["direct"][0],
+                  "fmm_s": fres["fmm"][0],
+                  "speedup_fmm_vs_direct": fres["direct"][0] / fres["fmm"][0],
+                  "rel_diff_fmm_vs_direct": float(
+                      np.linalg.norm(fres["fmm"][1] - fres["direct"][1]) /
+                      np.linalg.norm(fres["direct"][1]))}
+            if solve:
+                import oracle as O
+                mief = O.mie_rigid_field(k, w["radius"], fpts, dirs[0])
+                for meth in ("direct", "fmm"):
+                    fr["rel_err_%s_vs_mie" % meth] = float(
+                        np.linalg.norm(fres[meth][1] - mief) / np.linalg.norm(mief))
+            field["r%.1fa" % rf] = fr
+            log("field %d points at %.2f m: direct %.3fs, through the tree %.3fs" % (
+                a.field_points, fr["radius_m"], fr["direct_s"], fr["fmm_s"]))
     for sv in solve.values():
         sv.pop("x", None)
     del p64
