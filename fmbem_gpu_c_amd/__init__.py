@@ -461,6 +461,8 @@ class Plan:
         expansions of the well-separated boxes, fmb_field_eval_fmm).
         Returns a [nrhs][M] complex128 CUDA tensor."""
         import torch
+        if method not in ("direct", "fmm"):
+            raise FmbError(-1, "field method is 'direct' or 'fmm'")
         nrhs = _vecs(x, self.n, "x")
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         d = None
@@ -469,8 +471,6 @@ class Plan:
             if d.shape[0] != nrhs:
                 raise FmbError(-1, "one incident direction per right-hand side")
         out = torch.empty((nrhs, pts.shape[0]), dtype=torch.complex128, device="cuda")
-        if method not in ("direct", "fmm"):
-            raise FmbError(-1, "field method is 'direct' or 'fmm'")
         fn = lib().fmb_field_eval if method == "direct" else lib().fmb_field_eval_fmm
         _check(fn(self.h, _dev(x), nrhs, ctypes.c_void_p(pts.ctypes.data), pts.shape[0],
                   ctypes.c_void_p(d.ctypes.data) if d is not None else None,

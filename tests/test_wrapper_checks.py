@@ -36,6 +36,7 @@ def test_vector_length_is_checked():
         _refused(p.gmres, bad)
         _refused(p.gmres_t, bad)
         _refused(p.field, bad, np.zeros((1, 3)))
+        _refused(p.field, bad, np.zeros((1, 3)), method="fmm")
         _refused(p.adjoint_grad, bad, np.zeros((1, 3)))
         _refused(p.sweep_guess, 1.0, bad)
 
@@ -77,3 +78,7 @@ def test_batch_rows_are_checked():
     _refused(F.gmres_batch, [a, b], [1, 1], _z(2, 10), x0=_z(3, 10))
     _refused(F.gmres_batch_refined, [a, b], [a], [1, 1], _z(2, 10))
     _refused(F.gmres_batch_refined, [a, b], [a, _plan(12)], [1, 1], _z(2, 10))
+
+
+def test_field_method_is_checked():
+    assert "method" in _refused(_plan().field, _z(10), np.zeros((1, 3)), method="tree")
