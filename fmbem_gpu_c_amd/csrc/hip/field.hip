@@ -4,7 +4,8 @@
 // with the same kernels, signs and panel rules as the collocation entries
 // (near.hip): the refined rule when |y - c_j| < ratio * radius_j.  The
 // reference has no field evaluation.  One workgroup per field point reduces
-// over all panels in FP64; R right-hand sides per pass.
+// over all panels in FP64 (for few points, over one slice of them, the
+// slices summed in order by k_field_sum); R right-hand sides per pass.
 //
 // Its transpose, for adjoint objectives: given weights u[r][q] at the field
 // points, per element j (caller order)
