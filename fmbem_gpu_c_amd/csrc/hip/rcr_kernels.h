@@ -34,7 +34,8 @@
 // (column-major, leading dimension = padded rows), so operand loads need no
 // guards: rows of X beyond a block are read at a clamped index and meet
 // zero operator entries.  Operands of the next KU K-steps are loaded into one
-// register set while the MFMAs consume the other.
+// register set while the MFMAs consume the other; operator blocks in global
+// memory through buffer loads (BUF), LDS copies (LDSO) through pointers.
 // Two workspace modes:
 //   LDS: X in LDS, updated in place; each block is owned by one wave (LPT
 //        schedule built on the host), which keeps the block's outputs in its
@@ -107,6 +108,39 @@ template <typename T, int NTL> struct Ahead {
 
 __device__ __forceinline__ int pad16(int s) { return (s + 15) & ~15; }
 
+// Operator blocks in global memory are read by buffer loads: the block's
+// base in a scalar resource (raw, stride 0, a 2 GB range, the gfx9 data
+// format word), the lane's byte offset in one VGPR for the whole K loop, the
+// K-step's byte offset as a scalar.  This spares the 64-bit address
+// arithmetic of a pointer load per operand (fewer VALU instructions; the
+// level-7 FP32 M2L 3% faster).  The offsets stay those of the pointer
+// loads, inside the padded blocks.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t op_rsrc(const void *p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7fffffff,
+                                           0x00020000);
+}
+template <typename V> struct OpLoad;
+template <> struct OpLoad<float> {
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+};
+template <> struct OpLoad<double> {
+  static __device__ __forceinline__ double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+  }
+};
+template <> struct OpLoad<cplx<float>> {
+  static __device__ __forceinline__ cplx<float> ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(cplx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+  }
+};
+template <> struct OpLoad<cplx<double>> {
+  static __device__ __forceinline__ cplx<double> ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(cplx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+  }
+};
+
 struct RcrArgs {
   const int4 *chunks;  // {bucket, first pair, end pair, 0}
   int nchunks, nb;     // chunks, batches per chunk (grid y / item stride)
@@ -151,7 +185,7 @@ __device__ __forceinline__ int eo_block(int s) { return ((s + 15) & ~15) * ((s +
 // and no more K-steps than the even one; its missing tiles and K-steps are
 // skipped (wave-uniform) and its operands read at clamped, in-bounds
 // positions.
-template <typename T, int NT, int NTL, int DIR>
+template <typename T, int NT, int NTL, int DIR, bool BUF = false>
 __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict__ Ab,
                                          int n, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -164,6 +198,10 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
   const T *ape = Ab + (unsigned)(lk * lde + 16 * t0 + li);
   // (a unit past the odd block's tiles reads the even block's operands)
   const T *apo = nto > 0 ? Ab + (unsigned)(eo_block(se) + lk * ldo + 16 * t0 + li) : ape;
+  constexpr int TS = (int)sizeof(T);
+  const __amdgpu_buffer_rsrc_t ar = op_rsrc(Ab);
+  const int vae = TS * (lk * lde + 16 * t0 + li);
+  const int vao = nto > 0 ? TS * (eo_block(se) + lk * ldo + 16 * t0 + li) : vae;
   // set by the first K-step (every block has one), which multiplies into a
   // zero accumulator operand instead of cleared registers
   acc_t acce[NTL][NT], acco[NTL][NT];
@@ -187,13 +225,23 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
           fbo[u][t] = q;
         }
       }
-      const T *ak = ape + (unsigned)(4 * ks * lde);
-#pragma unroll
-      for (int mt = 0; mt < NTL; mt++) fae[u][mt] = ak[16 * mt];
       const int kso = min(ks, max(nko - 1, 0));
-      const T *ao = apo + (unsigned)(4 * kso * ldo);
+      if constexpr (BUF) {
+        const int ske = uni(TS * 4 * ks * lde), sko = uni(TS * 4 * kso * ldo);
 #pragma unroll
-      for (int mt = 0; mt < NTL; mt++) fao[u][mt] = ao[16 * (mt < nto ? mt : 0)];
+        for (int mt = 0; mt < NTL; mt++)
+          fae[u][mt] = OpLoad<T>::ld(ar, vae + TS * 16 * mt, ske);
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++)
+          fao[u][mt] = OpLoad<T>::ld(ar, vao + TS * 16 * (mt < nto ? mt : 0), sko);
+      } else {
+        const T *ak = ape + (unsigned)(4 * ks * lde);
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++) fae[u][mt] = ak[16 * mt];
+        const T *ao = apo + (unsigned)(4 * kso * ldo);
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++) fao[u][mt] = ao[16 * (mt < nto ? mt : 0)];
+      }
     }
   };
   auto mma = [&](int g, const T(&fae)[KU][NTL], const T(&fao)[KU][NTL],
@@ -273,7 +321,7 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
 // share the matrix C^{am}: rows (n', +-am), n' = am .. pout-1, from rows
 // (n, +-am), n = am .. pin-1.  Ab: the block (padded, leading dimension
 // pad16(so)).
-template <typename T, int NT, int NTL, int SG>
+template <typename T, int NT, int NTL, int SG, bool BUF = false>
 __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
                                           const cplx<T> *__restrict__ Ab,
                                           int am, int pin, int pout, int t0) {
@@ -285,6 +333,9 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
   const int so = pout - am, si = pin - am;
   const int ld = pad16(so), nks = (si + 3) >> 2, ng = (nks + KU - 1) / KU;
   const cplx<T> *ap = Ab + (unsigned)(lk * ld + 16 * t0 + li);
+  constexpr int CS = (int)sizeof(cplx<T>);
+  const __amdgpu_buffer_rsrc_t ar = op_rsrc(Ab);
+  const int va = CS * (lk * ld + 16 * t0 + li);
   // set by the first K-step (every block has one), as in rot_unit
   acc_t acc[NTL][NB];
   cplx<T> a0[KU][NTL], z0[KU][NB], a1[KU][NTL], z1[KU][NB];
@@ -299,9 +350,15 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
 #pragma unroll
         for (int t = 0; t < NT; t++) fz[u][sg * NT + t] = *(const cplx<T> *)(xr + 16 * t);
       }
-      const cplx<T> *ak = ap + (unsigned)(4 * ks * ld);
+      if constexpr (BUF) {
+        const int sk = uni(CS * 4 * ks * ld);
 #pragma unroll
-      for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
+        for (int mt = 0; mt < NTL; mt++) fa[u][mt] = OpLoad<cplx<T>>::ld(ar, va + CS * 16 * mt, sk);
+      } else {
+        const cplx<T> *ak = ap + (unsigned)(4 * ks * ld);
+#pragma unroll
+        for (int mt = 0; mt < NTL; mt++) fa[u][mt] = ak[16 * mt];
+      }
     }
   };
   auto mma = [&](int g, const cplx<T>(&fa)[KU][NTL], const cplx<T>(&fz)[KU][NB]) {
@@ -369,41 +426,41 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
 }
 
 // runs unit<NTL = ntl> for a wave-uniform ntl in [1, MT]
-template <typename T, int NT, int MT, int DIR>
+template <typename T, int NT, int MT, int DIR, bool BUF = false>
 __device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo, const T *Ab, int n,
                                         int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      rot_any<T, NT, MT - 1, DIR>(ntl, Xi, Xo, Ab, n, t0);
+      rot_any<T, NT, MT - 1, DIR, BUF>(ntl, Xi, Xo, Ab, n, t0);
       return;
     }
   }
-  rot_unit<T, NT, MT, DIR>(Xi, Xo, Ab, n, t0);
+  rot_unit<T, NT, MT, DIR, BUF>(Xi, Xo, Ab, n, t0);
 }
 
-template <typename T, int NT, int MT, int SG>
+template <typename T, int NT, int MT, int SG, bool BUF = false>
 __device__ __forceinline__ void coax_any(int ntl, const T *Xi, T *Xo,
                                          const cplx<T> *Ab, int am, int pin,
                                          int pout, int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      coax_any<T, NT, MT - 1, SG>(ntl, Xi, Xo, Ab, am, pin, pout, t0);
+      coax_any<T, NT, MT - 1, SG, BUF>(ntl, Xi, Xo, Ab, am, pin, pout, t0);
       return;
     }
   }
-  coax_unit<T, NT, MT, SG>(Xi, Xo, Ab, am, pin, pout, t0);
+  coax_unit<T, NT, MT, SG, BUF>(Xi, Xo, Ab, am, pin, pout, t0);
 }
 
 // Rotation stage over this wave's units {n*16 + g, operator offset}: row
 // tiles [g*MT, g*MT + MT) of degree n's even and odd blocks.
-template <typename T, int NT, int MT, int DIR>
+template <typename T, int NT, int MT, int DIR, bool BUF = false>
 __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
                                           const int *units, int count) {
   for (int j = 0; j < count; j++) {
     const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
     const int n = code >> 4, t0 = (code & 15) * MT;
     const int ntl = min(MT, ((n + 1 + 15) >> 4) - t0);
-    rot_any<T, NT, MT, DIR>(ntl, Xi, Xo, H + aoff, n, t0);
+    rot_any<T, NT, MT, DIR, BUF>(ntl, Xi, Xo, H + aoff, n, t0);
   }
 }
 
@@ -526,7 +583,7 @@ __device__ __forceinline__ void rot_stage_bf(const float *Xi, float *Xo, const _
 
 // Coaxial stage over this wave's units {am*16 + g, operator offset}: the
 // blocks +am and -am together.
-template <typename T, int NT, int MT>
+template <typename T, int NT, int MT, bool BUF = false>
 __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
                                            const int *units, int count, int pin,
                                            int pout) {
@@ -535,9 +592,9 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
     const int am = code >> 4, t0 = (code & 15) * MT;
     const int ntl = min(MT, ((pout - am + 15) >> 4) - t0);
     if (am == 0)
-      coax_any<T, NT, MT, 1>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
+      coax_any<T, NT, MT, 1, BUF>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
     else
-      coax_any<T, NT, MT, 2>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
+      coax_any<T, NT, MT, 2, BUF>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
   }
 }
 
@@ -636,15 +693,15 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
     const __bf16 *Hb = (const __bf16 *)a.Hb + a.hbstride * oi.beta;
     rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT>(X1, X0, C, uc, wcc, pin, pout);
+    coax_stage<T, NT, MT, !OPL>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
     rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u3, w3c);
   } else {
-    rot_stage<T, NT, MT, 0>(X0, X1, H, u1, w1c);
+    rot_stage<T, NT, MT, 0, !OPL>(X0, X1, H, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT>(X1, X0, C, uc, wcc, pin, pout);
+    coax_stage<T, NT, MT, !OPL>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
-    rot_stage<T, NT, MT, 1>(X0, X1, H, u3, w3c);
+    rot_stage<T, NT, MT, 1, !OPL>(X0, X1, H, u3, w3c);
   }
   __syncthreads();
   // add e^{-i m' phi} y into the targets.  Thread e adds real e of each
