@@ -111,6 +111,7 @@ typedef struct fmbh_rcr_job {
                         contiguous range of the (chunk, batch) items      */
   const void *in;
   void *out;
+  size_t in_bytes, out_bytes; /* extents of in and out this pass reaches */
   void *gws;
   int gws_grid;
 } fmbh_rcr_job;

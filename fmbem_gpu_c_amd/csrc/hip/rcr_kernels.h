@@ -609,9 +609,13 @@ __device__ __forceinline__ long long lane64(long long x, int l) {
 // pre() runs just before the first barrier (the LDS-mode kernel stores its
 // schedule there, so that the schedule's loads and the source gather are in
 // flight together).
-// OPL: the bucket's operators are Hs / Cs (LDS copies) instead of global
+// OPL: the bucket's operators are Hs / Cs (LDS copies) instead of global.
+// B32 (FP32 only): the source and target arrays span under 4 GB, so the
+// gather reads them by buffer loads and the scatter adds by buffer atomics,
+// one scalar byte offset per vector and one VGPR offset per lane, instead
+// of 64-bit pointer arithmetic per access.
 template <typename T, int NT, int MT, int W, bool GWS, bool BF = false, bool OPL = false,
-          typename Pre>
+          bool B32 = false, typename Pre>
 __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
                                           T *X0, T *X1, const int *sched, Pre pre,
                                           const T *Hs = nullptr,
@@ -648,9 +652,19 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
     soff = ((long long)a.psrc[q] * a.nrhs + r) * Pi2;
     toff = ((long long)a.ptgt[q] * a.nrhs + r) * Po2;
   }
+  constexpr int CS = (int)sizeof(cplx<T>);
   const cplx<T> *vin[V];
+  int sin[V];
+  __amdgpu_buffer_rsrc_t rin;
+  if constexpr (B32) {
+    rin = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.Min), (short)0, -1,
+                                            0x00020000);
 #pragma unroll
-  for (int v = 0; v < V; v++) vin[v] = (const cplx<T> *)a.Min + lane64(soff, v);
+    for (int v = 0; v < V; v++) sin[v] = (int)(unsigned)(lane64(soff, v) * CS);
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) vin[v] = (const cplx<T> *)a.Min + lane64(soff, v);
+  }
   // source coefficients times e^{i m phi} (times 1/2 for the even/odd
   // rotations, whose first stage forms z_m + z_{-m} and z_m - z_{-m}); rows
   // f >= pin^2 are zero.  Rows f = tid and f = tid + NTH are loaded (at
@@ -659,8 +673,13 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   constexpr T kScale = BF ? T(1) : T(0.5);
   auto row = [&](int f, cplx<T> (&z)[V], cplx<T> &w, int &m) {
     const int g = min(f, Pi2 - 1), h = min(f, Pm2 - 1);
+    if constexpr (B32) {
 #pragma unroll
-    for (int v = 0; v < V; v++) z[v] = vin[v][(unsigned)g];
+      for (int v = 0; v < V; v++) z[v] = OpLoad<cplx<T>>::ld(rin, g * CS, sin[v]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) z[v] = vin[v][(unsigned)g];
+    }
     const int n = isqrt_fix(h);
     m = h - n * n - n;
     w = phb[m < 0 ? -m : m];
@@ -709,8 +728,16 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
   // atomic adds fill whole cache lines: the adds are performed beyond the
   // L2, and half-filled lines cost nearly as much as full ones.
   T *vout[V];
+  int sout[V];
+  __amdgpu_buffer_rsrc_t rout;
+  if constexpr (B32) {
+    rout = __builtin_amdgcn_make_buffer_rsrc(a.Lout, (short)0, -1, 0x00020000);
 #pragma unroll
-  for (int v = 0; v < V; v++) vout[v] = (T *)((cplx<T> *)a.Lout + lane64(toff, v));
+    for (int v = 0; v < V; v++) sout[v] = (int)(unsigned)(lane64(toff, v) * CS);
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) vout[v] = (T *)((cplx<T> *)a.Lout + lane64(toff, v));
+  }
   for (int e = tid; e < 2 * Po2; e += NTH) {
     const int f = e >> 1, n = isqrt_fix(f), m = f - n * n - n;
     const bool im = e & 1;
@@ -721,7 +748,10 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
       if (v < nv) {
         const cplx<T> x = *(const cplx<T> *)(X1 + f * RS + 2 * v);
         const T y = im ? x.re * w.im + x.im * w.re : x.re * w.re - x.im * w.im;
-        unsafeAtomicAdd(vout[v] + (unsigned)e, y);
+        if constexpr (B32)
+          __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(y, rout, e * (int)sizeof(T), sout[v], 0);
+        else
+          unsafeAtomicAdd(vout[v] + (unsigned)e, y);
       }
   }
 }
@@ -732,7 +762,7 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
 template <typename T, int W> constexpr int kWavesPerEu = sizeof(T) == 4 ? (W == 8 ? 4 : 3) : 1;
 
 // LDS mode: grid (nchunks, nb), one batch per workgroup.
-template <typename T, int NT, int MT, int W, bool BF = false>
+template <typename T, int NT, int MT, int W, bool BF = false, bool B32 = false>
 __global__ void __launch_bounds__(kThreads<W>)
     __attribute__((amdgpu_waves_per_eu(kWavesPerEu<T, W>))) k_rcr_lds(RcrArgs a) {
   constexpr int V = 8 * NT, RS = Row<NT>::stride;
@@ -761,7 +791,7 @@ __global__ void __launch_bounds__(kThreads<W>)
   constexpr int NTH = kThreads<W>;
   const int t = threadIdx.x, n = a.sched_len;
   const int s0 = t < n ? a.sched[t] : 0, s1 = t + NTH < n ? a.sched[t + NTH] : 0;
-  rcr_batch<T, NT, MT, W, false, BF>(a, ch, u0, X, X, sched, [&] {
+  rcr_batch<T, NT, MT, W, false, BF, false, B32>(a, ch, u0, X, X, sched, [&] {
     if (t < n) sched[t] = s0;
     if (t + NTH < n) sched[t + NTH] = s1;
     for (int i = t + 2 * NTH; i < n; i += NTH) sched[i] = a.sched[i];
@@ -939,6 +969,10 @@ int launch(const fmbh_rcr_job *j, int nrhs, hipStream_t st) {
   if (j->mode == FMBH_RCR_LDS) {
     const size_t lds = x_bytes(Pm, NT, tsz) + small_lds(Pm, tsz, j->sched_len);
     auto kern = k_rcr_lds<T, NT, MT, W, BF>;
+    // FP32 arrays under 4 GB: the buffer-addressed gather and scatter
+    if constexpr (sizeof(T) == 4 && !BF)
+      if (j->in_bytes < (1ull << 32) && j->out_bytes < (1ull << 32))
+        kern = k_rcr_lds<T, NT, MT, W, BF, true>;
     int rc = set_lds(kern, lds);
     if (rc) return rc;
     if (a.xcd) {

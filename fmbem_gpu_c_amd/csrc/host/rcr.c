@@ -880,6 +880,12 @@ void fmbx_rcr_free(fmb_plan *pl) {
   }
 }
 
+/* bytes of level l's coefficient array that a pass over nrhs vectors of
+ * order p reaches ([box][nrhs][p^2]) */
+static size_t coef_bytes(const fmb_plan *pl, const fmb_opset *o, int l, int p, int nrhs) {
+  return (size_t)pl->tree->nbox[l] * nrhs * p * p * o->csz;
+}
+
 /* kind 0: M2L at level l; 1: M2M level l -> l-1; 2: L2L level l-1 -> l */
 int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
                  void *stream) {
@@ -939,6 +945,9 @@ int fmbx_rcr_run(const fmb_plan *pl, int op, int kind, int l, int nrhs,
     j.in = o->d_L[l - 1];
     j.out = o->d_L[l];
   }
+  /* source level: l (M2L, M2M) or l - 1 (L2L); target level: l or l - 1 */
+  j.in_bytes = coef_bytes(pl, o, kind == 2 ? l - 1 : l, j.pin, nrhs);
+  j.out_bytes = coef_bytes(pl, o, kind == 1 ? l - 1 : l, j.pout, nrhs);
   return fmbh_rcr(&j, o->prec, nrhs, stream);
 }
 
@@ -1009,6 +1018,9 @@ int fmbx_rcr_run_t(const fmb_plan *pl, int op, int kind, int l, int nrhs,
     j.in = o->d_L[l];
     j.out = o->d_L[l - 1];
   }
+  /* source level: l (M2L^T, L2L^T) or l - 1 (M2M^T); target: l or l - 1 */
+  j.in_bytes = coef_bytes(pl, o, kind == 1 ? l - 1 : l, j.pin, nrhs);
+  j.out_bytes = coef_bytes(pl, o, kind == 2 ? l - 1 : l, j.pout, nrhs);
   return fmbh_rcr(&j, o->prec, nrhs, stream);
 }
 
