@@ -5,6 +5,8 @@ minutes of CPU time):
 
 * the FP32 pass against the FP64 pass, 1e-6 relative;
 * the transpose identity y . (A x) = (A^T y) . x, 1e-12 of |y| |A x|;
+* a 26-vector FP32 pass (level-7 arrays over 4 GB, pointer-addressed
+  translations there) against the two-vector pass, 1e-6;
 * two FP64 matvecs in a row, 1e-13 (target atomics change the order of the
   sums only);
 * GMRES to 1e-6 in both work precisions: the solutions agree to 1e-5, and
@@ -56,6 +58,26 @@ def test_full_size_matvec_properties(sphere7):
         scale = np.linalg.norm(y[r]) * np.linalg.norm(a64[r])
         print("rhs %d transpose identity %.2e" % (r, abs(lhs - rhs) / scale), flush=True)
         assert abs(lhs - rhs) < 1e-12 * scale
+
+
+def test_full_size_wide_batch_pointer_path(sphere7):
+    """26 right-hand sides: the FP32 level-7 coefficient arrays then span
+    4.45 GB, beyond the buffer addressing's 4 GB, so that level's
+    translations run the pointer-addressed kernel while the coarser levels
+    keep the buffer-addressed one.  The first two vectors' FP32 matvec
+    matches the two-vector plan's (buffer-addressed everywhere) to 1e-6."""
+    m, t, k, pl = sphere7
+    n = m.arrays()[1].shape[0]
+    assert t.info()[1] == 7 and 65984 * 26 * 18 * 18 * 8 > 2 ** 32
+    wide = F.Plan(t, k, precision=F.FMB_MIXED, nrhs_max=26, autotune=0)
+    x = _rand((26, n), 74)
+    xt = torch.tensor(x, device="cuda")
+    y26 = wide.matvec(xt, prec=F.FMB_MIXED).cpu().numpy()
+    y2 = pl.matvec(xt[:2].contiguous(), prec=F.FMB_MIXED).cpu().numpy()
+    err = np.linalg.norm(y26[:2] - y2) / np.linalg.norm(y2)
+    print("26-vector pass vs 2-vector pass (FP32) %.2e" % err, flush=True)
+    assert err < 1e-6
+    del wide
 
 
 def test_full_size_solve_vs_mie(sphere7):
