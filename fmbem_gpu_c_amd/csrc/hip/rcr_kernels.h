@@ -35,7 +35,9 @@
 // guards: rows of X beyond a block are read at a clamped index and meet
 // zero operator entries.  Operands of the next KU K-steps are loaded into one
 // register set while the MFMAs consume the other; operator blocks in global
-// memory through buffer loads (BUF), LDS copies (LDSO) through pointers.
+// memory through buffer loads (BUF), LDS copies (LDSO) through pointers;
+// the batch rows likewise by buffer loads where they sit in the global
+// workspace (XG, GWS mode), by LDS reads otherwise.
 // Two workspace modes:
 //   LDS: X in LDS, updated in place; each block is owned by one wave (LPT
 //        schedule built on the host), which keeps the block's outputs in its
@@ -185,7 +187,7 @@ __device__ __forceinline__ int eo_block(int s) { return ((s + 15) & ~15) * ((s +
 // and no more K-steps than the even one; its missing tiles and K-steps are
 // skipped (wave-uniform) and its operands read at clamped, in-bounds
 // positions.
-template <typename T, int NT, int NTL, int DIR, bool BUF = false>
+template <typename T, int NT, int NTL, int DIR, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict__ Ab,
                                          int n, int t0) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -202,6 +204,7 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
   const __amdgpu_buffer_rsrc_t ar = op_rsrc(Ab);
   const int vae = TS * (lk * lde + 16 * t0 + li);
   const int vao = nto > 0 ? TS * (eo_block(se) + lk * ldo + 16 * t0 + li) : vae;
+  const __amdgpu_buffer_rsrc_t xrs = op_rsrc(Xi);
   // set by the first K-step (every block has one), which multiplies into a
   // zero accumulator operand instead of cleared registers
   acc_t acce[NTL][NT], acco[NTL][NT];
@@ -214,9 +217,17 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
       const int ks = min(g * KU + u, nke - 1);
       const int k = min(4 * ks + lk, n), m = k < n ? k + 1 : 0;
       const T *xp = Xi + (c0 + m) * RS + li, *xm = Xi + (c0 - m) * RS + li;
+      const int xop = TS * ((c0 + m) * RS + li), xom = TS * ((c0 - m) * RS + li);
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const T p = xp[16 * t], q = xm[16 * t];
+        T p, q;
+        if constexpr (XG) {
+          p = OpLoad<T>::ld(xrs, xop + TS * 16 * t, 0);
+          q = OpLoad<T>::ld(xrs, xom + TS * 16 * t, 0);
+        } else {
+          p = xp[16 * t];
+          q = xm[16 * t];
+        }
         if constexpr (DIR == 0) {
           fbe[u][t] = p + q;
           fbo[u][t] = p - q;
@@ -321,7 +332,7 @@ __device__ __forceinline__ void rot_unit(const T *Xi, T *Xo, const T *__restrict
 // share the matrix C^{am}: rows (n', +-am), n' = am .. pout-1, from rows
 // (n, +-am), n = am .. pin-1.  Ab: the block (padded, leading dimension
 // pad16(so)).
-template <typename T, int NT, int NTL, int SG, bool BUF = false>
+template <typename T, int NT, int NTL, int SG, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
                                           const cplx<T> *__restrict__ Ab,
                                           int am, int pin, int pout, int t0) {
@@ -336,6 +347,7 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
   constexpr int CS = (int)sizeof(cplx<T>);
   const __amdgpu_buffer_rsrc_t ar = op_rsrc(Ab);
   const int va = CS * (lk * ld + 16 * t0 + li);
+  const __amdgpu_buffer_rsrc_t xrs = op_rsrc(Xi);
   // set by the first K-step (every block has one), as in rot_unit
   acc_t acc[NTL][NB];
   cplx<T> a0[KU][NTL], z0[KU][NB], a1[KU][NTL], z1[KU][NB];
@@ -347,8 +359,14 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
 #pragma unroll
       for (int sg = 0; sg < SG; sg++) {
         const T *xr = Xi + (rb + (sg ? -am : am)) * RS + (li & ~1);
+        const int xo = (int)sizeof(T) * ((rb + (sg ? -am : am)) * RS + (li & ~1));
 #pragma unroll
-        for (int t = 0; t < NT; t++) fz[u][sg * NT + t] = *(const cplx<T> *)(xr + 16 * t);
+        for (int t = 0; t < NT; t++) {
+          if constexpr (XG)
+            fz[u][sg * NT + t] = OpLoad<cplx<T>>::ld(xrs, xo + (int)sizeof(T) * 16 * t, 0);
+          else
+            fz[u][sg * NT + t] = *(const cplx<T> *)(xr + 16 * t);
+        }
       }
       if constexpr (BUF) {
         const int sk = uni(CS * 4 * ks * ld);
@@ -426,41 +444,41 @@ __device__ __forceinline__ void coax_unit(const T *Xi, T *Xo,
 }
 
 // runs unit<NTL = ntl> for a wave-uniform ntl in [1, MT]
-template <typename T, int NT, int MT, int DIR, bool BUF = false>
+template <typename T, int NT, int MT, int DIR, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void rot_any(int ntl, const T *Xi, T *Xo, const T *Ab, int n,
                                         int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      rot_any<T, NT, MT - 1, DIR, BUF>(ntl, Xi, Xo, Ab, n, t0);
+      rot_any<T, NT, MT - 1, DIR, BUF, XG>(ntl, Xi, Xo, Ab, n, t0);
       return;
     }
   }
-  rot_unit<T, NT, MT, DIR, BUF>(Xi, Xo, Ab, n, t0);
+  rot_unit<T, NT, MT, DIR, BUF, XG>(Xi, Xo, Ab, n, t0);
 }
 
-template <typename T, int NT, int MT, int SG, bool BUF = false>
+template <typename T, int NT, int MT, int SG, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void coax_any(int ntl, const T *Xi, T *Xo,
                                          const cplx<T> *Ab, int am, int pin,
                                          int pout, int t0) {
   if constexpr (MT > 1) {
     if (ntl < MT) {
-      coax_any<T, NT, MT - 1, SG, BUF>(ntl, Xi, Xo, Ab, am, pin, pout, t0);
+      coax_any<T, NT, MT - 1, SG, BUF, XG>(ntl, Xi, Xo, Ab, am, pin, pout, t0);
       return;
     }
   }
-  coax_unit<T, NT, MT, SG, BUF>(Xi, Xo, Ab, am, pin, pout, t0);
+  coax_unit<T, NT, MT, SG, BUF, XG>(Xi, Xo, Ab, am, pin, pout, t0);
 }
 
 // Rotation stage over this wave's units {n*16 + g, operator offset}: row
 // tiles [g*MT, g*MT + MT) of degree n's even and odd blocks.
-template <typename T, int NT, int MT, int DIR, bool BUF = false>
+template <typename T, int NT, int MT, int DIR, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void rot_stage(const T *Xi, T *Xo, const T *H,
                                           const int *units, int count) {
   for (int j = 0; j < count; j++) {
     const int code = uni(units[2 * j]), aoff = uni(units[2 * j + 1]);
     const int n = code >> 4, t0 = (code & 15) * MT;
     const int ntl = min(MT, ((n + 1 + 15) >> 4) - t0);
-    rot_any<T, NT, MT, DIR, BUF>(ntl, Xi, Xo, H + aoff, n, t0);
+    rot_any<T, NT, MT, DIR, BUF, XG>(ntl, Xi, Xo, H + aoff, n, t0);
   }
 }
 
@@ -583,7 +601,7 @@ __device__ __forceinline__ void rot_stage_bf(const float *Xi, float *Xo, const _
 
 // Coaxial stage over this wave's units {am*16 + g, operator offset}: the
 // blocks +am and -am together.
-template <typename T, int NT, int MT, bool BUF = false>
+template <typename T, int NT, int MT, bool BUF = false, bool XG = false>
 __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
                                            const int *units, int count, int pin,
                                            int pout) {
@@ -592,9 +610,9 @@ __device__ __forceinline__ void coax_stage(const T *Xi, T *Xo, const cplx<T> *C,
     const int am = code >> 4, t0 = (code & 15) * MT;
     const int ntl = min(MT, ((pout - am + 15) >> 4) - t0);
     if (am == 0)
-      coax_any<T, NT, MT, 1, BUF>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
+      coax_any<T, NT, MT, 1, BUF, XG>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
     else
-      coax_any<T, NT, MT, 2, BUF>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
+      coax_any<T, NT, MT, 2, BUF, XG>(ntl, Xi, Xo, C + aoff, am, pin, pout, t0);
   }
 }
 
@@ -712,15 +730,15 @@ __device__ __forceinline__ void rcr_batch(const RcrArgs &a, int4 ch, int u0,
     const __bf16 *Hb = (const __bf16 *)a.Hb + a.hbstride * oi.beta;
     rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT, !OPL>(X1, X0, C, uc, wcc, pin, pout);
+    coax_stage<T, NT, MT, !OPL, GWS>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
     rot_stage_bf<NT, MT>(X0, X1, Hb, a.hbhalf, u3, w3c);
   } else {
-    rot_stage<T, NT, MT, 0, !OPL>(X0, X1, H, u1, w1c);
+    rot_stage<T, NT, MT, 0, !OPL, GWS>(X0, X1, H, u1, w1c);
     __syncthreads();
-    coax_stage<T, NT, MT, !OPL>(X1, X0, C, uc, wcc, pin, pout);
+    coax_stage<T, NT, MT, !OPL, GWS>(X1, X0, C, uc, wcc, pin, pout);
     __syncthreads();
-    rot_stage<T, NT, MT, 1, !OPL>(X0, X1, H, u3, w3c);
+    rot_stage<T, NT, MT, 1, !OPL, GWS>(X0, X1, H, u3, w3c);
   }
   __syncthreads();
   // add e^{-i m' phi} y into the targets.  Thread e adds real e of each
