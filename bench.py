@@ -160,6 +160,22 @@ def traffic_of(workload, lbig, shape):
     return lv["fetch_kb"] * 1024 * d["fetch_scale"] + lv["write_kb"] * 1024 * d["write_scale"]
 
 
+def mfma_busy_of(workload, lbig, shape):
+    """Matrix-core busy fraction of the dominant kernel from the same
+    committed counter record (SQ_VALU_MFMA_BUSY_CYCLES over the launch's
+    cycles on all SIMDs), or None without a record for this level and tile
+    shape."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    lv = d.get("levels", {}).get(str(lbig))
+    if not lv or lv.get("tile_shape") != list(shape):
+        return None
+    return lv.get("mfma_busy")
+
+
 def plan_kwargs(a):
     return dict(autotune=0 if a.no_autotune else 1, serial_levels=1 if a.serial_levels else 0,
                 split_bf16=1 if a.split_bf16 else 0, ldso=1 if a.ldso else 0,
@@ -232,6 +248,7 @@ def run_sphere(a, w, torch, F, log):
     ach64 = flops[lbig] / (m2l64[lbig] * 1e-3) / 1e12
     shape32 = plan.tile_shape_bf(0, lbig, F.FMB_MIXED)[:5]
     traffic = traffic_of(a.workload, lbig, shape32)
+    mfma_busy = mfma_busy_of(a.workload, lbig, shape32)
 
     solve = {}
     field = None
@@ -390,6 +407,9 @@ def run_sphere(a, w, torch, F, log):
                      "traffic": traffic,
                      # HBM rate of that launch: its counted bytes over its time
                      "hbm_GBps": (traffic / (m2l32[lbig] * 1e-3) / 1e9) if traffic else None,
+                     # matrix-core busy fraction of that launch (rocprofv3 SQ
+                     # counters, committed beside the traffic record)
+                     "mfma_busy": mfma_busy,
                      "kernel": "M2L level %d, FP32 matrix-core kernel" % lbig,
                      "tile_shape": list(shape32),
                      "fp64_achieved": ach64, "fp64_frac": ach64 / PEAK["fp64"],
